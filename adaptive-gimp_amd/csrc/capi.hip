@@ -1,0 +1,502 @@
+// C-ABI entry points (include/agimp.h): device state of one simulator and the host-side
+// orchestration of a substep (adaptive_gimp.py:323-352).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/agimp.h"
+#include "common.hpp"
+#include "launch.hpp"
+#include "svd.hpp"
+
+using namespace agimp;
+
+struct agimp_handle {
+  agimp_config cfg{};
+  int D = 0, L = 0;
+  int n = 0;
+  hipStream_t stream = nullptr;
+  Topo T{};
+  Physics ph{};
+  PBuf pa{}, pb{};  // current / next particle buffers
+  SortBufs S{};
+  unsigned long long* counters = nullptr;
+  uint32_t* vflag = nullptr;  // block flags of valid nodes of the current octree
+  int8_t* st_ret = nullptr;
+  int8_t* st_act = nullptr;
+  int static_mode = 0;
+  int has_particles = 0;
+  int have_slists = 0;     // grid blocks of the last substep are listed (to be cleared)
+  int topo_valid = 0;      // masks describe the current octree
+  int CH = 256;
+  int max_chunks = 0;
+  int p2g_grid = 0, grid_grid = 0;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+namespace {
+
+int fail(agimp_handle* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+#define HIPCHK(h, call)                                                                          \
+  do {                                                                                           \
+    hipError_t e_ = (call);                                                                      \
+    if (e_ != hipSuccess) return fail((h), AGIMP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class Tp>
+int dalloc(agimp_handle* h, Tp** p, size_t count) {
+  void* q = nullptr;
+  size_t bytes = std::max<size_t>(count * sizeof(Tp), 256);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return fail(h, AGIMP_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  e = hipMemsetAsync(q, 0, bytes, h->stream);
+  if (e != hipSuccess) return fail(h, AGIMP_EHIP, hipGetErrorString(e));
+  h->allocs.push_back(q);
+  *p = (Tp*)q;
+  return 0;
+}
+
+int align_size(int x, int a) { return (x + (a - 1)) & ~(a - 1); }  // utils.py:44-45
+
+Physics make_physics(const agimp_config& c) {
+  Physics p{};
+  for (int a = 0; a < 3; ++a) p.gravity[a] = c.gravity[a];
+  p.boundary_gap = c.boundary_gap;
+  p.wall_mode = c.wall_mode;
+  p.plastic = c.plastic;
+  p.theta_c = c.theta_c;
+  p.theta_s = c.theta_s;
+  p.hardening = c.hardening;
+  double E = c.E, nu = c.nu;
+  p.mu0 = E / (2.0 * (1.0 + nu));                    // adaptive_gimp.py:53
+  p.la0 = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  p.p_vol = c.p_vol;
+  p.slab_enable = c.slab_enable;
+  p.slab_axis = c.slab_axis;
+  p.slab_lo = c.slab_lo;
+  p.slab_hi = c.slab_hi;
+  return p;
+}
+
+int check_ready(agimp_handle* h) {
+  if (!h) return AGIMP_EINVAL;
+  if (!h->has_particles) return fail(h, AGIMP_ESTATE, "particles not set (agimp_set_particles)");
+  return 0;
+}
+
+size_t flag_words(const Topo& T) {
+  size_t flags = 0;
+  for (int l = 0; l < T.L; ++l) flags += (size_t)T.lv[l].nblocks;
+  return (flags + 3) / 4;
+}
+
+// zero the grid blocks the previous physics step wrote, and empty the block lists
+void clear_prev(agimp_handle* h) {
+  if (h->have_slists) launch_clear_sgrid(h->stream, h->T, h->grid_grid);
+  launch_fill_u32(h->stream, h->T.scount, 0, MAXL);
+  h->have_slists = 0;
+}
+
+int do_refine(agimp_handle* h) {
+  Topo& T = h->T;
+  clear_prev(h);
+  // cell masks of all levels are one allocation, as are the block flags
+  launch_fill_u32(h->stream, T.lv[0].cell, 0, cells_total(T, 0, T.L - 1) / 4);
+  launch_fill_u32(h->stream, T.node, 0, (size_t)T.lv[T.L - 1].nblocks * (h->D == 2 ? 16 : 64) / 4);
+  launch_fill_u32(h->stream, T.lv[0].sflag, 0, flag_words(T));
+  launch_refine(h->stream, T, h->pa, h->n, h->static_mode, h->st_ret, h->st_act, h->counters);
+  // blocks holding valid nodes of the new octree, kept for fixed-topology substeps
+  launch_copy_u32(h->stream, h->vflag, T.lv[0].sflag, flag_words(T));
+  h->topo_valid = 1;
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int do_physics(agimp_handle* h, float dt) {
+  Topo& T = h->T;
+  clear_prev(h);
+  launch_copy_u32(h->stream, T.lv[0].sflag, h->vflag, flag_words(T));
+  launch_fill_u32(h->stream, h->S.bcnt, 0, (size_t)T.total_blocks);
+  launch_fill_u32(h->stream, h->S.frozen, 0, 1);
+  launch_bucket(h->stream, T, h->pa, h->n, h->S, h->CH, h->counters);
+  Physics ph = h->ph;
+  ph.dt = dt;
+  launch_physics(h->stream, T, h->pa, h->pb, h->n, h->S, ph, h->p2g_grid, h->grid_grid, h->counters);
+  std::swap(h->pa, h->pb);
+  h->have_slists = 1;
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int agimp_config_init(agimp_config* c, int32_t dim, int32_t level, int32_t coarsest_size, int64_t n_particles) {
+  if (!c) return AGIMP_EINVAL;
+  std::memset(c, 0, sizeof(*c));
+  c->dim = dim;
+  c->level = level;
+  c->coarsest_size = coarsest_size;
+  c->n_particles = n_particles;
+  c->boundary_gap = 4;
+  c->E = 5e3f;
+  c->nu = 0.2f;
+  c->gravity[1] = -100.0f;
+  c->p_vol = 1.0f;
+  c->radius = 0.5f;
+  c->wall_mode = 0;
+  c->plastic = 0;
+  c->theta_c = 2.5e-2f;
+  c->theta_s = 7.5e-3f;
+  c->hardening = 0.0f;
+  c->slab_axis = 1;
+  c->particles_per_chunk = 256;
+  c->device = 0;
+  return 0;
+}
+
+int agimp_create(const agimp_config* cfg, agimp_handle** out) {
+  if (!cfg || !out) return AGIMP_EINVAL;
+  *out = nullptr;
+  const agimp_config& c = *cfg;
+  if (c.dim != 2 && c.dim != 3) return AGIMP_EINVAL;
+  if (c.level < 1 || c.level > MAXL) return AGIMP_EINVAL;
+  if (c.coarsest_size < 1 || c.n_particles < 1 || c.n_particles > (int64_t)1 << 30) return AGIMP_EINVAL;
+  if (std::fabs(c.radius - 0.5f) > 0.0f) return AGIMP_EINVAL;  // B-spline form needs r = 0.5
+  if (c.particles_per_chunk < 16 || c.particles_per_chunk > 65536) return AGIMP_EINVAL;
+  if (c.slab_enable && (c.slab_axis < 0 || c.slab_axis >= c.dim)) return AGIMP_EINVAL;
+  agimp_handle* h = new agimp_handle();
+  h->cfg = c;
+  h->D = c.dim;
+  h->L = c.level;
+  h->n = (int)c.n_particles;
+  h->CH = c.particles_per_chunk;
+  int rc = 0;
+  if (hipSetDevice(c.device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return AGIMP_EHIP;
+  }
+  Topo& T = h->T;
+  T.dim = c.dim;
+  T.L = c.level;
+  T.c = c.coarsest_size;
+  T.finest_size = c.coarsest_size << (c.level - 1);  // adaptive_gimp.py:17
+  T.pow2 = (c.coarsest_size & (c.coarsest_size - 1)) == 0;
+  const int p4 = c.dim == 2 ? 16 : 64;
+  size_t cells = 0, flags = 0;
+  int gbase = 0;
+  for (int l = 0; l < T.L; ++l) {
+    LevelDesc& Ll = T.lv[l];
+    Ll.n = c.coarsest_size << l;
+    Ll.e = 4 * align_size(Ll.n / 4 + 1, 4);  // adaptive_gimp.py:36-37
+    Ll.E = Ll.e + 2 * PAD;
+    Ll.NB = Ll.E / 4;
+    Ll.shift = T.L - 1 - l;
+    Ll.nblocks = (c.dim == 2) ? Ll.NB * Ll.NB : Ll.NB * Ll.NB * Ll.NB;
+    Ll.gbase = gbase;
+    gbase += Ll.nblocks;
+    // coarsest_dx / 2**l in f32 (adaptive_gimp.py:16, :235)
+    Ll.dx = (float)(1.0 / c.coarsest_size) / (float)(1 << l);
+    Ll.inv_dx = (float)(c.coarsest_size << l);
+    cells += (size_t)Ll.nblocks * p4;
+    flags += (size_t)Ll.nblocks;
+  }
+  T.total_blocks = gbase;
+  uint8_t* cellbuf = nullptr;
+  float4* gridbuf = nullptr;
+  uint8_t* flagbuf = nullptr;
+  int* slistbuf = nullptr;
+  rc = rc ? rc : dalloc(h, &cellbuf, (cells + 3) & ~(size_t)3);
+  rc = rc ? rc : dalloc(h, &gridbuf, cells);
+  rc = rc ? rc : dalloc(h, &flagbuf, (flags + 3) & ~(size_t)3);
+  rc = rc ? rc : dalloc(h, &slistbuf, flags);
+  rc = rc ? rc : dalloc(h, &h->vflag, (flags + 3) / 4);
+  rc = rc ? rc : dalloc(h, &T.node, (size_t)T.lv[T.L - 1].nblocks * p4);
+  rc = rc ? rc : dalloc(h, &T.scount, MAXL);
+  if (rc) { agimp_destroy(h); return rc; }
+  size_t co = 0, fo = 0;
+  for (int l = 0; l < T.L; ++l) {
+    T.lv[l].cell = cellbuf + co;
+    T.lv[l].grid = gridbuf + co;
+    T.lv[l].sflag = flagbuf + fo;
+    T.lv[l].slist = slistbuf + fo;
+    co += (size_t)T.lv[l].nblocks * p4;
+    fo += (size_t)T.lv[l].nblocks;
+  }
+  // particles
+  const int NF = (c.dim == 2) ? PLayout<2>::NF : PLayout<3>::NF;
+  int64_t stride = ((int64_t)h->n + 63) & ~(int64_t)63;
+  for (PBuf* pbuf : {&h->pa, &h->pb}) {
+    pbuf->stride = stride;
+    rc = rc ? rc : dalloc(h, &pbuf->f, (size_t)NF * stride);
+    rc = rc ? rc : dalloc(h, &pbuf->g, stride);
+    rc = rc ? rc : dalloc(h, &pbuf->id, stride);
+  }
+  SortBufs& S = h->S;
+  rc = rc ? rc : dalloc(h, &S.gb, stride);
+  rc = rc ? rc : dalloc(h, &S.rk, stride);
+  rc = rc ? rc : dalloc(h, &S.perm, stride);
+  rc = rc ? rc : dalloc(h, &S.flv, stride);
+  rc = rc ? rc : dalloc(h, &S.bcnt, T.total_blocks);
+  rc = rc ? rc : dalloc(h, &S.boff, T.total_blocks);
+  rc = rc ? rc : dalloc(h, &S.choff, T.total_blocks);
+  rc = rc ? rc : dalloc(h, &S.tiles, (T.total_blocks + 2047) / 2048 + 1);
+  rc = rc ? rc : dalloc(h, &S.totals, 4);
+  rc = rc ? rc : dalloc(h, &S.frozen, 4);
+  h->max_chunks = (int)std::min<int64_t>((int64_t)h->n / h->CH + std::min<int64_t>(h->n, T.total_blocks) + 1,
+                                         (int64_t)1 << 30);
+  rc = rc ? rc : dalloc(h, &S.chunks, h->max_chunks);
+  rc = rc ? rc : dalloc(h, &h->counters, NCOUNTERS);
+  if (rc) { agimp_destroy(h); return rc; }
+  h->ph = make_physics(c);
+  h->p2g_grid = std::max(1, std::min(h->max_chunks, 8192));
+  h->grid_grid = 4096;
+  if (hipStreamSynchronize(h->stream) != hipSuccess) { agimp_destroy(h); return AGIMP_EHIP; }
+  *out = h;
+  return 0;
+}
+
+int agimp_destroy(agimp_handle* h) {
+  if (!h) return AGIMP_EINVAL;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->st_ret) (void)hipFree(h->st_ret);
+  if (h->st_act) (void)hipFree(h->st_act);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int agimp_set_particles(agimp_handle* h, const float* x, const float* v, const float* F, const float* C,
+                        const float* m, const int32_t* g, const float* Jp) {
+  if (!h || !x || !v || !F || !C || !m || !g) return AGIMP_EINVAL;
+  const int D = h->D, n = h->n;
+  size_t nv = (size_t)n * D, nm = (size_t)n * D * D;
+  float *dx, *dv, *dF, *dC, *dm, *dJ = nullptr;
+  int* dg;
+  size_t bytes = (2 * nv + 2 * nm + 2 * (size_t)n) * sizeof(float) + (size_t)n * sizeof(int);
+  char* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, bytes));
+  dx = (float*)tmp;
+  dv = dx + nv;
+  dF = dv + nv;
+  dC = dF + nm;
+  dm = dC + nm;
+  dJ = dm + n;
+  dg = (int*)(dJ + n);
+  hipError_t e = hipSuccess;
+  e = e ? e : hipMemcpyAsync(dx, x, nv * 4, hipMemcpyHostToDevice, h->stream);
+  e = e ? e : hipMemcpyAsync(dv, v, nv * 4, hipMemcpyHostToDevice, h->stream);
+  e = e ? e : hipMemcpyAsync(dF, F, nm * 4, hipMemcpyHostToDevice, h->stream);
+  e = e ? e : hipMemcpyAsync(dC, C, nm * 4, hipMemcpyHostToDevice, h->stream);
+  e = e ? e : hipMemcpyAsync(dm, m, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
+  e = e ? e : hipMemcpyAsync(dg, g, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
+  if (Jp && e == hipSuccess) e = hipMemcpyAsync(dJ, Jp, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) {
+    launch_upload(h->stream, D, h->pa, n, dx, dv, dF, dC, dm, dg, Jp ? dJ : nullptr, h->S.flv);
+    launch_stress(h->stream, D, h->pa, n, h->ph);
+    e = hipStreamSynchronize(h->stream);
+  }
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("set_particles: ") + hipGetErrorString(e));
+  h->has_particles = 1;
+  return 0;
+}
+
+int agimp_set_static_levels(agimp_handle* h, const int8_t* ret_level, const int8_t* act_level) {
+  if (!h) return AGIMP_EINVAL;
+  if (!ret_level || !act_level) {
+    h->static_mode = 0;
+    return 0;
+  }
+  size_t E = (size_t)h->T.lv[h->L - 1].E;
+  size_t cnt = (h->D == 2) ? E * E : E * E * E;
+  for (size_t i = 0; i < cnt; ++i)
+    if (ret_level[i] < 0 || ret_level[i] >= h->L || act_level[i] < -1 || act_level[i] >= h->L)
+      return fail(h, AGIMP_EINVAL, "static level map entry outside [0, level)");
+  if (!h->st_ret) {
+    HIPCHK(h, hipMalloc(&h->st_ret, cnt));
+    HIPCHK(h, hipMalloc(&h->st_act, cnt));
+  }
+  HIPCHK(h, hipMemcpyAsync(h->st_ret, ret_level, cnt, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync(h->st_act, act_level, cnt, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->static_mode = 1;
+  return 0;
+}
+
+int agimp_refine(agimp_handle* h) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  return do_refine(h);
+}
+
+int agimp_substep(agimp_handle* h, float dt) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  rc = do_refine(h);
+  if (rc) return rc;
+  return do_physics(h, dt);
+}
+
+int agimp_substep_fixed_topology(agimp_handle* h, float dt) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!h->topo_valid) return fail(h, AGIMP_ESTATE, "no octree yet: call agimp_refine or agimp_substep first");
+  return do_physics(h, dt);
+}
+
+int agimp_advance(agimp_handle* h, float dt, int32_t n) {
+  if (n < 0) return AGIMP_EINVAL;
+  for (int i = 0; i < n; ++i) {
+    int rc = agimp_substep(h, dt);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int agimp_get_particles(agimp_handle* h, float* x, float* v, float* F, float* C, float* Jp, int32_t* fl,
+                        int32_t* g) {
+  if (!h) return AGIMP_EINVAL;
+  const int D = h->D, n = h->n;
+  size_t nv = (size_t)n * D, nm = (size_t)n * D * D;
+  size_t bytes = (2 * nv + 2 * nm + (size_t)n) * sizeof(float) + 2 * (size_t)n * sizeof(int);
+  char* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, bytes));
+  float* dx = (float*)tmp;
+  float* dv = dx + nv;
+  float* dF = dv + nv;
+  float* dC = dF + nm;
+  float* dJ = dC + nm;
+  int* dfl = (int*)(dJ + n);
+  int* dg = dfl + n;
+  launch_download(h->stream, D, h->pa, n, h->S.flv, dx, dv, dF, dC, dJ, dfl, dg);
+  hipError_t e = hipGetLastError();
+  if (x && e == hipSuccess) e = hipMemcpyAsync(x, dx, nv * 4, hipMemcpyDeviceToHost, h->stream);
+  if (v && e == hipSuccess) e = hipMemcpyAsync(v, dv, nv * 4, hipMemcpyDeviceToHost, h->stream);
+  if (F && e == hipSuccess) e = hipMemcpyAsync(F, dF, nm * 4, hipMemcpyDeviceToHost, h->stream);
+  if (C && e == hipSuccess) e = hipMemcpyAsync(C, dC, nm * 4, hipMemcpyDeviceToHost, h->stream);
+  if (Jp && e == hipSuccess) e = hipMemcpyAsync(Jp, dJ, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
+  if (fl && e == hipSuccess) e = hipMemcpyAsync(fl, dfl, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
+  if (g && e == hipSuccess) e = hipMemcpyAsync(g, dg, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("get_particles: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int agimp_get_level(agimp_handle* h, int32_t l, int32_t* cell_mask, float* m, float* v) {
+  if (!h || l < 0 || l >= h->L) return AGIMP_EINVAL;
+  size_t tot = 1;
+  for (int a = 0; a < h->D; ++a) tot *= (size_t)h->T.lv[l].e;
+  char* tmp = nullptr;
+  HIPCHK(h, hipMalloc(&tmp, tot * (4 + 4 + 4 * (size_t)h->D)));
+  int* dc = (int*)tmp;
+  float* dm = (float*)(dc + tot);
+  float* dv = dm + tot;
+  launch_dense_level(h->stream, h->T, l, dc, dm, dv);
+  hipError_t e = hipGetLastError();
+  if (cell_mask && e == hipSuccess) e = hipMemcpyAsync(cell_mask, dc, tot * 4, hipMemcpyDeviceToHost, h->stream);
+  if (m && e == hipSuccess) e = hipMemcpyAsync(m, dm, tot * 4, hipMemcpyDeviceToHost, h->stream);
+  if (v && e == hipSuccess) e = hipMemcpyAsync(v, dv, tot * 4 * h->D, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("get_level: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int agimp_get_node_mask(agimp_handle* h, int32_t* out) {
+  if (!h || !out) return AGIMP_EINVAL;
+  size_t tot = 1;
+  for (int a = 0; a < h->D; ++a) tot *= (size_t)h->T.lv[h->L - 1].e;
+  int* d = nullptr;
+  HIPCHK(h, hipMalloc(&d, tot * 4));
+  launch_dense_node(h->stream, h->T, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(out, d, tot * 4, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("get_node_mask: ") + hipGetErrorString(e));
+  return 0;
+}
+
+int agimp_get_counters(agimp_handle* h, int64_t* out, int32_t n) {
+  if (!h || !out || n < 0) return AGIMP_EINVAL;
+  unsigned long long tmp[NCOUNTERS];
+  HIPCHK(h, hipMemcpyAsync(tmp, h->counters, sizeof(tmp), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (int i = 0; i < n && i < NCOUNTERS; ++i) out[i] = (int64_t)tmp[i];
+  return 0;
+}
+
+int agimp_reset_counters(agimp_handle* h) {
+  if (!h) return AGIMP_EINVAL;
+  launch_fill_u32(h->stream, h->counters, 0, NCOUNTERS * 2);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int agimp_level_extent(agimp_handle* h, int32_t l) {
+  if (!h || l < 0 || l >= h->L) return AGIMP_EINVAL;
+  return h->T.lv[l].e;
+}
+
+int agimp_sync(agimp_handle* h) {
+  if (!h) return AGIMP_EINVAL;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+const char* agimp_last_error(agimp_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int agimp_host_svd(int32_t dim, const double* F, double* U, double* s, double* V) {
+  if (!F || !U || !s || !V) return AGIMP_EINVAL;
+  if (dim == 2) {
+    double Fl[2][2], Ul[2][2], Vl[2][2];
+    for (int i = 0; i < 4; ++i) Fl[i / 2][i % 2] = F[i];
+    svd2(Fl, Ul, s, Vl);
+    for (int i = 0; i < 4; ++i) { U[i] = Ul[i / 2][i % 2]; V[i] = Vl[i / 2][i % 2]; }
+    return 0;
+  }
+  if (dim == 3) {
+    double Fl[3][3], Ul[3][3], Vl[3][3];
+    for (int i = 0; i < 9; ++i) Fl[i / 3][i % 3] = F[i];
+    svd3(Fl, Ul, s, Vl);
+    for (int i = 0; i < 9; ++i) { U[i] = Ul[i / 3][i % 3]; V[i] = Vl[i / 3][i % 3]; }
+    return 0;
+  }
+  return AGIMP_EINVAL;
+}
+
+int agimp_host_constitutive(const agimp_config* cfg, double* F, double* Jp, double* tau, int32_t clamp) {
+  if (!cfg || !F || !Jp || !tau) return AGIMP_EINVAL;
+  Physics ph = make_physics(*cfg);
+  if (cfg->dim == 2) {
+    double Fl[2][2], T[2][2];
+    for (int i = 0; i < 4; ++i) Fl[i / 2][i % 2] = F[i];
+    constitutive<2>(Fl, *Jp, ph, T, clamp != 0);
+    for (int i = 0; i < 4; ++i) { F[i] = Fl[i / 2][i % 2]; tau[i] = T[i / 2][i % 2]; }
+    return 0;
+  }
+  if (cfg->dim == 3) {
+    double Fl[3][3], T[3][3];
+    for (int i = 0; i < 9; ++i) Fl[i / 3][i % 3] = F[i];
+    constitutive<3>(Fl, *Jp, ph, T, clamp != 0);
+    for (int i = 0; i < 9; ++i) { F[i] = Fl[i / 3][i % 3]; tau[i] = T[i / 3][i % 3]; }
+    return 0;
+  }
+  return AGIMP_EINVAL;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+// Shared definitions for the adaptive-GIMP substep on gfx950.
+//
+// Memory layout in HBM (one simulator handle):
+//  * every level l keeps dense, block-major arrays over its padded index box
+//    [-PAD, e_l + PAD)^d, e_l = 4*align_size(n_l/4 + 1, 4) (adaptive_gimp.py:36-37): a 4^d leaf
+//    block (adaptive_gimp.py:28) is contiguous (64 nodes x 16 B = 1 KiB in 3D), blocks are
+//    row-major.  Cell flags are one byte per cell, grid nodes one float4 (v.xyz, m).
+//  * node_mask (adaptive_gimp.py:23-26) is one byte per finest node in the finest layout.
+//  * particles are structure-of-arrays, one float array per component, kept in the order
+//    of the last substep's (level, leaf block) sort; orig_id maps back to caller order.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace agimp {
+
+constexpr int MAXL = 6;  // octree levels supported
+constexpr int PAD = 4;   // padding cells on each side of every level (one leaf block)
+
+// cell / node flag bits, utils.py:5-9
+constexpr uint8_t ACTIVATED = 1, GHOST = 2, T_JUNCTION = 4, CULLING = 8;
+
+struct LevelDesc {
+  int n;        // cells per axis in the domain, c * 2^l
+  int e;        // reference extent per axis
+  int E;        // padded extent per axis (e + 2*PAD)
+  int NB;       // leaf blocks per axis (E / 4)
+  int shift;    // L-1-l: level-l index -> finest index multiplier exponent
+  int nblocks;  // NB^d
+  int gbase;    // first global block id of this level
+  float dx, inv_dx;
+  uint8_t* cell;   // [nblocks * 4^d]
+  float4* grid;    // [nblocks * 4^d]
+  uint8_t* sflag;  // [nblocks]   block must be processed by the grid phase
+  int* slist;      // [nblocks]   compacted sflag list
+};
+
+struct Topo {
+  int dim, L, c;
+  int finest_size;
+  int total_blocks;  // sum of nblocks over levels
+  int pow2;          // coarsest size is a power of two: x/dx == x*inv_dx exactly
+  LevelDesc lv[MAXL];
+  uint8_t* node;     // node_mask, finest layout
+  int* scount;       // [MAXL] number of entries in each slist
+};
+
+struct Physics {
+  float gravity[3];
+  float dt;
+  int boundary_gap;
+  int wall_mode;  // 0: per-axis normal zeroing (adaptive_gimp.py:284-287), 1: sticky
+  int plastic;    // 0 none, 1 singular-value clamp
+  double theta_c, theta_s, hardening;
+  double mu0, la0;
+  float p_vol;
+  int slab_enable, slab_axis;
+  float slab_lo, slab_hi;
+};
+
+// Particle structure-of-arrays: component c of particle slot k is f[c*stride + k].
+template <int D>
+struct PLayout {
+  static constexpr int X = 0, V = D, F = 2 * D, C = 2 * D + D * D, TAU = 2 * D + 2 * D * D;
+  static constexpr int NTAU = D * (D + 1) / 2;
+  static constexpr int M = TAU + NTAU, JP = M + 1, NF = JP + 1;
+};
+// tau packing (symmetric): 2D xx,yy,xy ; 3D xx,yy,zz,xy,xz,yz
+template <int D>
+__host__ __device__ inline int tau_idx(int i, int j) {
+  if (i == j) return i;
+  if (D == 2) return 2;
+  int a = i < j ? i : j, b = i < j ? j : i;
+  return (a == 0) ? (b == 1 ? 3 : 4) : 5;
+}
+
+struct PBuf {
+  float* f;       // [NF * stride]
+  int* g;         // [stride] prescribed level g_p
+  int* id;        // [stride] original particle index
+  int64_t stride;
+};
+
+// counters kept on the device (int64 each)
+enum Counter : int {
+  CNT_OOB = 0,        // accesses outside the padded level boxes (reference: undefined behaviour)
+  CNT_FROZEN = 1,     // particles with fl_p = -1 this substep
+  CNT_ACTIVE = 2,     // particles transferred this substep
+  CNT_CHUNKS = 3,     // P2G work chunks
+  CNT_NAN = 4,        // particles whose new state is not finite
+  NCOUNTERS = 8
+};
+
+// block-major index of padded coordinates J (each in [0, E))
+template <int D>
+__device__ __forceinline__ uint32_t bidx(int NB, const int* J) {
+  if (D == 2) {
+    uint32_t b = (uint32_t)((J[0] >> 2) * NB + (J[1] >> 2));
+    return (b << 4) | (uint32_t)(((J[0] & 3) << 2) | (J[1] & 3));
+  } else {
+    uint32_t b = (uint32_t)(((J[0] >> 2) * NB + (J[1] >> 2)) * NB + (J[2] >> 2));
+    return (b << 6) | (uint32_t)(((J[0] & 3) << 4) | ((J[1] & 3) << 2) | (J[2] & 3));
+  }
+}
+
+template <int D>
+__device__ __forceinline__ bool in_box(int E, const int* J) {
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) ok = ok && (J[a] >= 0) && (J[a] < E);
+  return ok;
+}
+
+// block id -> block coordinates
+template <int D>
+__device__ __forceinline__ void block_coords(int NB, int b, int* B) {
+  if (D == 2) { B[0] = b / NB; B[1] = b - B[0] * NB; }
+  else { B[2] = b % NB; int r = b / NB; B[1] = r % NB; B[0] = r / NB; }
+}
+
+// the floor((x/dx + 0.5)) - 1 base cell of a particle at level l, with the reference's
+// f32 arithmetic: x/dx exact for power-of-two sizes, then +0.5 and truncation toward zero
+__device__ __forceinline__ int base_cell(float x, const LevelDesc& L, int pow2) {
+  float q = pow2 ? x * L.inv_dx : x / L.dx;
+  return (int)(q + 0.5f) - 1;
+}
+__device__ __forceinline__ float scaled(float x, const LevelDesc& L, int pow2) {
+  return pow2 ? x * L.inv_dx : x / L.dx;
+}
+
+__device__ __forceinline__ int level_of_gbid(const Topo& T, int gbid) {
+  int l = 0;
+#pragma unroll
+  for (int q = 1; q < MAXL; ++q)
+    if (q < T.L && gbid >= T.lv[q].gbase) l = q;
+  return l;
+}
+
+// ---------------------------------------------------------------------------------------
+// Run-aggregated atomic add: lanes whose (active, addr) equal their left neighbour's join
+// its run; each run's first lane adds the run length once and every lane receives
+// base + its offset in the run.  Must be called by all lanes of the wavefront.
+__device__ __forceinline__ int wave_run_atomic_add(int* addr, bool active) {
+  const int lane = __lane_id();
+  unsigned long long a_bits = (unsigned long long)(uintptr_t)addr;
+  unsigned long long prev = __shfl_up(a_bits, 1);
+  int prev_act = __shfl_up((int)active, 1);
+  bool head = !active || lane == 0 || !prev_act || prev != a_bits;
+  unsigned long long heads = __ballot(head);
+  unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  int h = 63 - __clzll(heads & le);
+  unsigned long long above = heads & ~((h == 63) ? ~0ull : ((1ull << (h + 1)) - 1ull));
+  int nh = above ? (__ffsll((long long)above) - 1) : 64;
+  int base = 0;
+  if (active && lane == h) base = atomicAdd(addr, nh - h);
+  base = __shfl(base, h);
+  return base + (lane - h);
+}
+
+}  // namespace agimp

@@ -1,0 +1,1195 @@
+// HIP kernels of the adaptive-GIMP substep for gfx950 (wave64).
+//
+// Phase order per substep (adaptive_gimp.py:323-352):
+//   refine:   clear -> activation (dynamic :87-96 | static :72-85) -> culling (:98-124)
+//             -> node A marks (:142-153) -> T marks -> ghost marks (:160-187)
+//   bucket:   level mapping (:189-216) as a counting sort by (level, leaf block)
+//   physics:  P2G (:230-250) -> restriction (:289-301) -> grid update (:277-287)
+//             -> prolongation (:303-321) -> G2P (:252-275) + plasticity + next stress
+//
+// Differences of execution (not of results) from the reference:
+//  * culled cells keep their A|C bits instead of being zeroed (every later test is an exact
+//    `== ACTIVATED` comparison, so A|C behaves as 0); readback strips them;
+//  * all T marks of all levels are made before any ghost mark, so the ghost test
+//    `node == UNACTIVATED` (:182) never races a T mark (:174): raw bits are canonical;
+//  * restriction and prolongation are written as gathers over the receiving node (same
+//    sums, no float atomics); prolongation overwrites instead of zero-then-add (:308-321);
+//  * the grid update skips nodes that prolongation overwrites anyway (T, G, all-even
+//    nodes on levels >= 1);
+//  * the Kirchhoff stress (utils.py:11-16) of the next P2G is computed at the end of G2P
+//    from the same FP64 SVD that the plasticity clamp needs, and stored (6 floats in 3D).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "launch.hpp"
+#include "svd.hpp"
+
+namespace agimp {
+
+template <int D>
+struct Pow4 { static constexpr int v = (D == 2) ? 16 : 64; };
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[i] = v;
+}
+
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) dst[i] = src[i];
+}
+
+// decode a flat cell/node index over the concatenated per-level arrays
+template <int D>
+__device__ __forceinline__ bool decode_flat(const Topo& T, int lmin, int lmax, size_t t, int& l, int* I, uint32_t& local) {
+  size_t acc = 0;
+  for (int q = 0; q < T.L; ++q) {
+    if (q < lmin || q > lmax) continue;
+    size_t cnt = (size_t)T.lv[q].nblocks * Pow4<D>::v;
+    if (t < acc + cnt) {
+      l = q;
+      local = (uint32_t)(t - acc);
+      int b = (int)(local / Pow4<D>::v);
+      int w = (int)(local % Pow4<D>::v);
+      int B[3];
+      block_coords<D>(T.lv[q].NB, b, B);
+      if (D == 2) {
+        I[0] = B[0] * 4 + (w >> 2) - PAD;
+        I[1] = B[1] * 4 + (w & 3) - PAD;
+      } else {
+        I[0] = B[0] * 4 + (w >> 4) - PAD;
+        I[1] = B[1] * 4 + ((w >> 2) & 3) - PAD;
+        I[2] = B[2] * 4 + (w & 3) - PAD;
+      }
+      return true;
+    }
+    acc += cnt;
+  }
+  return false;
+}
+
+// index of level-l coordinates I (unpadded), or -1 if outside the padded box
+template <int D>
+__device__ __forceinline__ int64_t lidx(const LevelDesc& L, const int* I) {
+  int J[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) J[a] = I[a] + PAD;
+  if (!in_box<D>(L.E, J)) return -1;
+  return (int64_t)bidx<D>(L.NB, J);
+}
+
+// node_mask entry of level-l node I (adaptive_gimp.py:126-128 maps to the finest index)
+template <int D>
+__device__ __forceinline__ int64_t nidx(const Topo& T, int l, const int* I) {
+  int f[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) f[a] = I[a] << T.lv[l].shift;
+  return lidx<D>(T.lv[T.L - 1], f);
+}
+
+__device__ __forceinline__ void count(unsigned long long* cnt, int which, unsigned long long v = 1) {
+  atomicAdd(&cnt[which], v);
+}
+
+// mark the grid-phase blocks of every level on which finest node f is a node
+template <int D>
+__device__ __forceinline__ void mark_valid_node(const Topo& T, const int* f) {
+  for (int l = T.L - 1; l >= 0; --l) {
+    int s = T.lv[l].shift;
+    int msk = (1 << s) - 1;
+    bool div = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) div = div && ((f[a] & msk) == 0);
+    if (!div) break;
+    int J[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) J[a] = (f[a] >> s) + PAD;
+    if (!in_box<D>(T.lv[l].E, J)) continue;
+    int b = (D == 2) ? ((J[0] >> 2) * T.lv[l].NB + (J[1] >> 2))
+                     : (((J[0] >> 2) * T.lv[l].NB + (J[1] >> 2)) * T.lv[l].NB + (J[2] >> 2));
+    T.lv[l].sflag[b] = 1;
+  }
+}
+
+// ------------------------------------------------------------------------------ clear
+template <int D>
+__global__ void k_clear_sgrid(Topo T) {
+  size_t total = 0;
+  for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += st) {
+    size_t acc = 0;
+    for (int l = 0; l < T.L; ++l) {
+      size_t cnt = (size_t)T.scount[l] * Pow4<D>::v;
+      if (t < acc + cnt) {
+        size_t q = t - acc;
+        int b = T.lv[l].slist[q / Pow4<D>::v];
+        T.lv[l].grid[(size_t)b * Pow4<D>::v + q % Pow4<D>::v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        break;
+      }
+      acc += cnt;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------ activation
+// adaptive_gimp.py:87-96: OR ACTIVATED into cell_mask[g_p][base + dI], dI in {0,1}^d
+template <int D>
+__global__ void k_activate_dynamic(Topo T, PBuf P, int n, unsigned long long* cnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int g = P.g[k];
+  if (g < 0 || g >= T.L) return;
+  const LevelDesc& Lg = T.lv[g];
+  int b[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) b[a] = base_cell(P.f[(size_t)a * P.stride + k], Lg, T.pow2);
+#pragma unroll
+  for (int t = 0; t < (1 << D); ++t) {
+    int I[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) I[a] = b[a] + ((t >> (D - 1 - a)) & 1);
+    int64_t q = lidx<D>(Lg, I);
+    if (q < 0) { count(cnt, CNT_OOB); continue; }
+    Lg.cell[q] = ACTIVATED;  // cells only ever receive ACTIVATED in this phase
+  }
+}
+
+// adaptive_gimp.py:72-85 with the grid initializer given as finest-cell level maps
+template <int D>
+__device__ __forceinline__ int static_gi(const Topo& T, const int8_t* ret, const int8_t* act, const int* I,
+                                         unsigned long long* cnt) {
+  const LevelDesc& F = T.lv[T.L - 1];
+  int J[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) J[a] = I[a] + PAD;
+  if (!in_box<D>(F.E, J)) { count(cnt, CNT_OOB); return T.L - 1; }
+  size_t r = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) r = r * F.E + J[a];
+  int al = act[r];
+  if (al >= 0 && al < T.L) {
+    int P[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) P[a] = I[a] >> (T.L - 1 - al);
+    int64_t q = lidx<D>(T.lv[al], P);
+    if (q < 0) count(cnt, CNT_OOB);
+    else T.lv[al].cell[q] = ACTIVATED;
+  }
+  return ret[r];
+}
+
+template <int D>
+__global__ void k_activate_static(Topo T, PBuf P, int n, const int8_t* ret, const int8_t* act,
+                                  unsigned long long* cnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const LevelDesc& F = T.lv[T.L - 1];
+  float x[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
+  int fb[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) fb[a] = base_cell(x[a], F, T.pow2);
+  constexpr int tot = (D == 2) ? 16 : 64;
+  for (int t = 0; t < tot; ++t) {
+    int I[3];
+    int r = t;
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) { I[a] = fb[a] + (r & 3) - 1; r >>= 2; }
+    static_gi<D>(T, ret, act, I, cnt);
+  }
+  int level = static_gi<D>(T, ret, act, fb, cnt);
+  if (level < 0 || level >= T.L) { count(cnt, CNT_OOB); return; }
+  int b[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], T.lv[level], T.pow2);
+#pragma unroll
+  for (int t = 0; t < (1 << D); ++t) {
+    int I[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) I[a] = (b[a] + ((t >> (D - 1 - a)) & 1)) << (T.L - 1 - level);
+    static_gi<D>(T, ret, act, I, cnt);
+  }
+}
+
+// ------------------------------------------------------------------------------ culling
+// adaptive_gimp.py:98-109 for all l >= 1 at once (ACTIVATED bits do not change here)
+template <int D>
+__global__ void k_mark_hier(Topo T, size_t total, unsigned long long* cnt) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int l, I[3];
+  uint32_t q;
+  if (!decode_flat<D>(T, 1, T.L - 1, t, l, I, q)) return;
+  if (!(T.lv[l].cell[q] & ACTIVATED)) return;
+  int l0 = -1;
+  for (int lq = 0; lq < l; ++lq) {
+    int P[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) P[a] = I[a] >> (l - lq);
+    int64_t qq = lidx<D>(T.lv[lq], P);
+    if (qq < 0) { count(cnt, CNT_OOB); continue; }
+    if (T.lv[lq].cell[qq] & ACTIVATED) l0 = lq;
+  }
+  if (l0 < 0) return;
+  for (int lq = l0; lq < l; ++lq) {
+    int P[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) P[a] = I[a] >> (l - lq);
+    int64_t qq = lidx<D>(T.lv[lq], P);
+    if (qq >= 0) T.lv[lq].cell[qq] |= CULLING;
+  }
+}
+
+// adaptive_gimp.py:111-118 for all l at once: children of CULLING cells get ACTIVATED.
+// The culled cell itself keeps A|C (it compares unequal to ACTIVATED from here on).
+template <int D>
+__global__ void k_split(Topo T, size_t total, unsigned long long* cnt) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int l, I[3];
+  uint32_t q;
+  if (!decode_flat<D>(T, 0, T.L - 2, t, l, I, q)) return;
+  if (!(T.lv[l].cell[q] & CULLING)) return;
+#pragma unroll
+  for (int u = 0; u < (1 << D); ++u) {
+    int Jc[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) Jc[a] = 2 * I[a] + ((u >> (D - 1 - a)) & 1);
+    int64_t qq = lidx<D>(T.lv[l + 1], Jc);
+    if (qq < 0) { count(cnt, CNT_OOB); continue; }
+    T.lv[l + 1].cell[qq] |= ACTIVATED;
+  }
+}
+
+// ------------------------------------------------------------------------------ node marks
+// adaptive_gimp.py:142-153: corners of exactly-ACTIVATED cells get ACTIVATED
+template <int D>
+__global__ void k_node_A(Topo T, size_t total, unsigned long long* cnt) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int l, I[3];
+  uint32_t q;
+  if (!decode_flat<D>(T, 0, T.L - 1, t, l, I, q)) return;
+  if (T.lv[l].cell[q] != ACTIVATED) return;
+#pragma unroll
+  for (int u = 0; u < (1 << D); ++u) {
+    int Jn[3], f[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) { Jn[a] = I[a] + ((u >> (D - 1 - a)) & 1); f[a] = Jn[a] << T.lv[l].shift; }
+    int64_t qq = lidx<D>(T.lv[T.L - 1], f);
+    if (qq < 0) { count(cnt, CNT_OOB); continue; }
+    T.node[qq] = ACTIVATED;  // node_mask only receives ACTIVATED in this phase
+    mark_valid_node<D>(T, f);
+  }
+}
+
+// shared face logic of adaptive_gimp.py:160-169; returns l_p or -1 if the face is skipped
+template <int D>
+__device__ __forceinline__ int face_parent(const Topo& T, int l, const int* I, int j, int* I1) {
+  int ax = j >> 1;
+#pragma unroll
+  for (int a = 0; a < D; ++a) I1[a] = I[a];
+  I1[ax] += (j & 1) ? -1 : 1;
+  bool inr = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) inr = inr && (I1[a] >= 0) && (I1[a] < T.lv[l].n);
+  if (!inr) return -1;
+  // _heriarchical_check :155-158
+  int lp = -1;
+  for (int lq = 0; lq <= l; ++lq) {
+    int P[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - lq);
+    int64_t qq = lidx<D>(T.lv[lq], P);
+    if (qq >= 0 && T.lv[lq].cell[qq] == ACTIVATED) lp = lq;
+  }
+  if (lp == l) return -1;  // cell_mask[l][I1] == ACTIVATED
+  return lp;
+}
+
+// adaptive_gimp.py:170-174 for all levels at once
+template <int D>
+__global__ void k_mark_T(Topo T, size_t total, unsigned long long* cnt) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int l, I[3];
+  uint32_t q;
+  if (!decode_flat<D>(T, 0, T.L - 1, t, l, I, q)) return;
+  if (T.lv[l].cell[q] != ACTIVATED) return;
+  for (int j = 0; j < 2 * D; ++j) {
+    int I1[3];
+    int lp = face_parent<D>(T, l, I, j, I1);
+    if (lp < 0) continue;
+    int ax = j >> 1, side = j & 1;
+    int msk = (1 << (l - lp)) - 1;
+#pragma unroll
+    for (int u = 0; u < (1 << D); ++u) {
+      int kk[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) kk[a] = (u >> (D - 1 - a)) & 1;
+      if (kk[ax] != side) continue;
+      bool any = false;
+      int f[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        int c = I1[a] + kk[a];
+        any = any || ((c & msk) != 0);
+        f[a] = c << T.lv[l].shift;
+      }
+      if (!any) continue;
+      int64_t qq = lidx<D>(T.lv[T.L - 1], f);
+      if (qq < 0) { count(cnt, CNT_OOB); continue; }
+      T.node[qq] |= T_JUNCTION;
+      mark_valid_node<D>(T, f);
+    }
+  }
+}
+
+// adaptive_gimp.py:176-183 for all levels at once, after every T mark
+template <int D>
+__global__ void k_mark_G(Topo T, size_t total, unsigned long long* cnt) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int l, I[3];
+  uint32_t q;
+  if (!decode_flat<D>(T, 0, T.L - 1, t, l, I, q)) return;
+  if (T.lv[l].cell[q] != ACTIVATED) return;
+  for (int j = 0; j < 2 * D; ++j) {
+    int I1[3];
+    int lp = face_parent<D>(T, l, I, j, I1);
+    if (lp < 0) continue;
+    for (int l0 = lp + 1; l0 <= l; ++l0) {
+      int P[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - l0);
+      int64_t qc = lidx<D>(T.lv[l0], P);
+      if (qc < 0) { count(cnt, CNT_OOB); continue; }
+      T.lv[l0].cell[qc] |= GHOST;
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        int f[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) f[a] = (P[a] + ((u >> (D - 1 - a)) & 1)) << T.lv[l0].shift;
+        int64_t qq = lidx<D>(T.lv[T.L - 1], f);
+        if (qq < 0) { count(cnt, CNT_OOB); continue; }
+        if (T.node[qq] == 0) {
+          T.node[qq] = GHOST;
+          mark_valid_node<D>(T, f);
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------ bucketing
+// adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
+// particle's 2^d cells; rank inside its (level, leaf block) bucket by run-aggregated atomics.
+template <int D>
+__global__ void k_levelmap(Topo T, PBuf P, int n, int* gb, int* rk, int* bcnt, int* frozen,
+                           unsigned long long* cnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = k < n;
+  int fl = -1, gbid = -1;
+  if (live) {
+    float x[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
+    for (int l = 0; l < T.L; ++l) {
+      const LevelDesc& Ll = T.lv[l];
+      int b[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Ll, T.pow2);
+      bool hit = false;
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        int I[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
+        int64_t q = lidx<D>(Ll, I);
+        hit = hit || (q >= 0 && Ll.cell[q] == ACTIVATED);
+      }
+      if (hit) fl = l;
+    }
+    if (fl >= 0) {
+      const LevelDesc& Lf = T.lv[fl];
+      int J[3];
+      bool ok = true;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        J[a] = base_cell(x[a], Lf, T.pow2) + PAD;
+        ok = ok && J[a] >= 0 && J[a] + 2 < Lf.E;
+      }
+      if (!ok) {
+        count(cnt, CNT_OOB);
+        fl = -1;
+      } else {
+        int b = (D == 2) ? ((J[0] >> 2) * Lf.NB + (J[1] >> 2))
+                         : (((J[0] >> 2) * Lf.NB + (J[1] >> 2)) * Lf.NB + (J[2] >> 2));
+        gbid = Lf.gbase + b;
+      }
+    }
+  }
+  int* addr = (gbid >= 0) ? &bcnt[gbid] : frozen;
+  int r = wave_run_atomic_add(addr, live);
+  if (live) {
+    gb[k] = gbid;
+    rk[k] = r;
+  }
+}
+
+// exclusive scan of (count, chunks) over all global blocks: 256 threads x 8 items per tile
+constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
+
+__device__ __forceinline__ int2 add2(int2 a, int2 b) { return make_int2(a.x + b.x, a.y + b.y); }
+
+__device__ __forceinline__ int2 block_excl_scan(int2 v, int2* sh, int2& total) {
+  int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_T; off <<= 1) {
+    int2 o = (tid >= off) ? sh[tid - off] : make_int2(0, 0);
+    __syncthreads();
+    sh[tid] = add2(sh[tid], o);
+    __syncthreads();
+  }
+  int2 incl = sh[tid];
+  total = sh[SCAN_T - 1];
+  __syncthreads();
+  return make_int2(incl.x - v.x, incl.y - v.y);
+}
+
+__device__ __forceinline__ int2 load_cc(const int* bcnt, int i, int n, int CH) {
+  if (i >= n) return make_int2(0, 0);
+  int c = bcnt[i];
+  return make_int2(c, (c + CH - 1) / CH);
+}
+
+__global__ void k_scan_reduce(const int* bcnt, int n, int CH, int2* tile_sum) {
+  __shared__ int2 sh[SCAN_T];
+  int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
+  int2 s = make_int2(0, 0);
+#pragma unroll
+  for (int i = 0; i < SCAN_I; ++i) s = add2(s, load_cc(bcnt, base + i, n, CH));
+  int2 tot;
+  block_excl_scan(s, sh, tot);
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
+}
+
+__global__ void k_scan_tiles(int2* tile_sum, int ntiles, int* totals) {
+  __shared__ int2 sh[SCAN_T];
+  int2 carry = make_int2(0, 0);
+  for (int base = 0; base < ntiles; base += SCAN_T) {
+    int i = base + threadIdx.x;
+    int2 v = (i < ntiles) ? tile_sum[i] : make_int2(0, 0);
+    int2 tot;
+    int2 ex = block_excl_scan(v, sh, tot);
+    if (i < ntiles) tile_sum[i] = add2(ex, carry);
+    carry = add2(carry, tot);
+  }
+  if (threadIdx.x == 0) { totals[0] = carry.x; totals[1] = carry.y; }
+}
+
+__global__ void k_scan_final(const int* bcnt, int n, int CH, const int2* tile_off, int* boff, int* choff) {
+  __shared__ int2 sh[SCAN_T];
+  int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
+  int2 v[SCAN_I];
+  int2 s = make_int2(0, 0);
+#pragma unroll
+  for (int i = 0; i < SCAN_I; ++i) { v[i] = load_cc(bcnt, base + i, n, CH); s = add2(s, v[i]); }
+  int2 tot;
+  int2 run = add2(block_excl_scan(s, sh, tot), tile_off[blockIdx.x]);
+#pragma unroll
+  for (int i = 0; i < SCAN_I; ++i) {
+    if (base + i < n) { boff[base + i] = run.x; choff[base + i] = run.y; }
+    run = add2(run, v[i]);
+  }
+}
+
+__global__ void k_scatter(int n, const int* gb, const int* rk, const int* boff, const int* totals, int* perm,
+                          unsigned long long* cnt) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int g = gb[k];
+  int pos = (g >= 0) ? boff[g] + rk[k] : totals[0] + rk[k];
+  perm[pos] = k;
+  if (k == 0) {
+    cnt[CNT_ACTIVE] += (unsigned long long)totals[0];
+    cnt[CNT_FROZEN] += (unsigned long long)(n - totals[0]);
+    cnt[CNT_CHUNKS] += (unsigned long long)totals[1];
+  }
+}
+
+// per occupied leaf block: P2G chunk descriptors + the blocks its 3^d stencils touch
+template <int D>
+__global__ void k_blocks(Topo T, const int* bcnt, const int* boff, const int* choff, int4* chunks, int CH) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= T.total_blocks) return;
+  int c = bcnt[g];
+  if (c == 0) return;
+  int l = level_of_gbid(T, g);
+  const LevelDesc& Ll = T.lv[l];
+  int b = g - Ll.gbase;
+  int nch = (c + CH - 1) / CH;
+  int o = choff[g], s = boff[g];
+  for (int j = 0; j < nch; ++j) chunks[o + j] = make_int4(g, s + j * CH, min(CH, c - j * CH), 0);
+  int B[3];
+  block_coords<D>(Ll.NB, b, B);
+#pragma unroll
+  for (int u = 0; u < (1 << D); ++u) {
+    int C2[3];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) { C2[a] = B[a] + ((u >> (D - 1 - a)) & 1); ok = ok && C2[a] < Ll.NB; }
+    if (!ok) continue;
+    int bb = (D == 2) ? (C2[0] * Ll.NB + C2[1]) : ((C2[0] * Ll.NB + C2[1]) * Ll.NB + C2[2]);
+    Ll.sflag[bb] = 1;
+  }
+}
+
+// compact the grid-phase block flags of every level into per-level lists
+template <int D>
+__global__ void k_scompact(Topo T) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = g < T.total_blocks;
+  int l = 0, b = 0;
+  bool on = false;
+  if (live) {
+    l = level_of_gbid(T, g);
+    b = g - T.lv[l].gbase;
+    on = T.lv[l].sflag[b] != 0;
+  }
+  int idx = wave_run_atomic_add(&T.scount[l], on);
+  if (on) T.lv[l].slist[idx] = b;
+}
+
+// ------------------------------------------------------------------------------ transfers
+// Quadratic B-spline weights and their t-derivatives: for r = 0.5 the GIMP overlap
+// integrals of utils.py:18-42 summed by adaptive_gimp.py:218-228 reduce to these.
+__device__ __forceinline__ void bspline(float t, float* w, float* dw) {
+  float a = 0.5f - t, c = 0.5f + t;
+  w[0] = 0.5f * a * a;
+  w[1] = 0.75f - t * t;
+  w[2] = 0.5f * c * c;
+  dw[0] = -a;
+  dw[1] = -2.0f * t;
+  dw[2] = c;
+}
+
+// adaptive_gimp.py:230-250, one work chunk = up to CH particles of one (level, leaf block);
+// each particle's 3^d contributions are summed in a (4+2)^d LDS node tile, flushed with
+// float atomics once per chunk.
+template <int D>
+__global__ void __launch_bounds__(256) k_p2g(Topo T, PBuf P, const int* perm, const int4* chunks,
+                                             const int* totals, Physics ph) {
+  using PL = PLayout<D>;
+  constexpr int TS = 6;
+  constexpr int TN = (D == 2) ? TS * TS : TS * TS * TS;
+  __shared__ float tile[TN * 4];
+  const int nchunks = totals[1];
+  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+    int4 ck = chunks[c];
+    int l = level_of_gbid(T, ck.x);
+    const LevelDesc& Ll = T.lv[l];
+    int B[3];
+    block_coords<D>(Ll.NB, ck.x - Ll.gbase, B);
+    for (int i = threadIdx.x; i < TN * 4; i += blockDim.x) tile[i] = 0.f;
+    __syncthreads();
+    const float scale = -ph.dt * Ll.inv_dx * ph.p_vol;  // -dt (0.5/r)^d / dx * V_p, r = 0.5
+    for (int i = threadIdx.x; i < ck.z; i += blockDim.x) {
+      int k = perm[ck.y + i];
+      float x[3], v[3], Cm[3][3], tau[6], mp;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        x[a] = P.f[(size_t)(PL::X + a) * P.stride + k];
+        v[a] = P.f[(size_t)(PL::V + a) * P.stride + k];
+      }
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int q = 0; q < D; ++q) Cm[a][q] = P.f[(size_t)(PL::C + a * D + q) * P.stride + k];
+#pragma unroll
+      for (int q = 0; q < PL::NTAU; ++q) tau[q] = P.f[(size_t)(PL::TAU + q) * P.stride + k];
+      mp = P.f[(size_t)PL::M * P.stride + k];
+      int lb[3];
+      float w[3][3], dw[3][3], tt[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        float qx = scaled(x[a], Ll, T.pow2);
+        int b = (int)(qx + 0.5f) - 1;
+        tt[a] = qx - (float)(b + 1);
+        lb[a] = b + PAD - 4 * B[a];
+        bspline(tt[a], w[a], dw[a]);
+      }
+      float S[3][3];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int q = 0; q < D; ++q) S[a][q] = scale * tau[tau_idx<D>(a, q)];
+      // v + C (i - (t+1)) = vb + C i
+      float vb[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        float acc = v[a];
+#pragma unroll
+        for (int q = 0; q < D; ++q) acc -= Cm[a][q] * (tt[q] + 1.0f);
+        vb[a] = acc;
+      }
+      if constexpr (D == 2) {
+#pragma unroll
+        for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+          for (int i1 = 0; i1 < 3; ++i1) {
+            float wt = w[0][i0] * w[1][i1];
+            float g0 = dw[0][i0] * w[1][i1], g1 = w[0][i0] * dw[1][i1];
+            float mw = wt * mp;
+            float p0 = mw * (vb[0] + Cm[0][0] * i0 + Cm[0][1] * i1) + S[0][0] * g0 + S[0][1] * g1;
+            float p1 = mw * (vb[1] + Cm[1][0] * i0 + Cm[1][1] * i1) + S[1][0] * g0 + S[1][1] * g1;
+            int nd = ((lb[0] + i0) * TS + (lb[1] + i1)) * 4;
+            atomicAdd(&tile[nd + 0], p0);
+            atomicAdd(&tile[nd + 1], p1);
+            atomicAdd(&tile[nd + 3], mw);
+          }
+      } else {
+#pragma unroll
+        for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+          for (int i1 = 0; i1 < 3; ++i1)
+#pragma unroll
+            for (int i2 = 0; i2 < 3; ++i2) {
+              float w01 = w[0][i0] * w[1][i1];
+              float wt = w01 * w[2][i2];
+              float g0 = dw[0][i0] * w[1][i1] * w[2][i2];
+              float g1 = w[0][i0] * dw[1][i1] * w[2][i2];
+              float g2 = w01 * dw[2][i2];
+              float mw = wt * mp;
+              float p[3];
+#pragma unroll
+              for (int a = 0; a < 3; ++a)
+                p[a] = mw * (vb[a] + Cm[a][0] * i0 + Cm[a][1] * i1 + Cm[a][2] * i2) + S[a][0] * g0 + S[a][1] * g1 +
+                       S[a][2] * g2;
+              int nd = (((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)) * 4;
+              atomicAdd(&tile[nd + 0], p[0]);
+              atomicAdd(&tile[nd + 1], p[1]);
+              atomicAdd(&tile[nd + 2], p[2]);
+              atomicAdd(&tile[nd + 3], mw);
+            }
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < TN; i += blockDim.x) {
+      float4 val = make_float4(tile[i * 4], tile[i * 4 + 1], tile[i * 4 + 2], tile[i * 4 + 3]);
+      if (val.x == 0.f && val.y == 0.f && val.z == 0.f && val.w == 0.f) continue;
+      int J[3];
+      if (D == 2) { J[0] = 4 * B[0] + i / TS; J[1] = 4 * B[1] + i % TS; }
+      else { J[0] = 4 * B[0] + i / (TS * TS); J[1] = 4 * B[1] + (i / TS) % TS; J[2] = 4 * B[2] + i % TS; }
+      if (!in_box<D>(Ll.E, J)) continue;
+      float4* dst = &Ll.grid[bidx<D>(Ll.NB, J)];
+      if (val.x != 0.f) unsafeAtomicAdd(&dst->x, val.x);
+      if (val.y != 0.f) unsafeAtomicAdd(&dst->y, val.y);
+      if (D == 3 && val.z != 0.f) unsafeAtomicAdd(&dst->z, val.z);
+      if (val.w != 0.f) unsafeAtomicAdd(&dst->w, val.w);
+    }
+    __syncthreads();
+  }
+}
+
+// node flags of level-l node I through the finest node_mask (0 outside the box)
+template <int D>
+__device__ __forceinline__ uint8_t nflag(const Topo& T, int l, const int* I) {
+  int64_t q = nidx<D>(T, l, I);
+  return q < 0 ? (uint8_t)0 : T.node[q];
+}
+__device__ __forceinline__ bool f_valid(uint8_t M) { return M > 0; }
+__device__ __forceinline__ bool f_act(uint8_t M) { return (M & (ACTIVATED | T_JUNCTION)) == ACTIVATED; }
+__device__ __forceinline__ bool f_T(uint8_t M) { return (M & T_JUNCTION) != 0; }
+__device__ __forceinline__ bool f_ghost(uint8_t M) { return (M & (ACTIVATED | GHOST | T_JUNCTION)) == GHOST; }
+
+// node I of list entry t of level l's grid-phase list
+template <int D>
+__device__ __forceinline__ uint32_t list_node(const LevelDesc& Ll, int e, int w, int* I) {
+  int b = Ll.slist[e];
+  int B[3];
+  block_coords<D>(Ll.NB, b, B);
+  if (D == 2) {
+    I[0] = B[0] * 4 + (w >> 2) - PAD;
+    I[1] = B[1] * 4 + (w & 3) - PAD;
+  } else {
+    I[0] = B[0] * 4 + (w >> 4) - PAD;
+    I[1] = B[1] * 4 + ((w >> 2) & 3) - PAD;
+    I[2] = B[2] * 4 + (w & 3) - PAD;
+  }
+  return (uint32_t)b * Pow4<D>::v + w;
+}
+
+template <int D>
+__device__ __forceinline__ bool in_range_nodes(const int* I, int n) {
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) ok = ok && I[a] >= 0 && I[a] <= n;
+  return ok;
+}
+
+// adaptive_gimp.py:289-301 as a gather onto each valid coarse node I of level l-1:
+// sources are fine nodes 2I - d with weight 2^-|d|_1.
+template <int D>
+__global__ void k_restrict(Topo T, int l) {
+  const LevelDesc& Lc = T.lv[l - 1];
+  const LevelDesc& Lf = T.lv[l];
+  int total = T.scount[l - 1] * Pow4<D>::v;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    int I[3];
+    uint32_t q = list_node<D>(Lc, t / Pow4<D>::v, t % Pow4<D>::v, I);
+    if (!in_range_nodes<D>(I, Lc.n)) continue;
+    if (!f_valid(nflag<D>(T, l - 1, I))) continue;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    constexpr int NS = (D == 2) ? 9 : 27;
+    bool any = false;
+    for (int s = 0; s < NS; ++s) {
+      int d[3], If[3], r = s, l1 = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) { d[a] = (r % 3) - 1; r /= 3; If[a] = 2 * I[a] - d[a]; l1 += d[a] != 0; }
+      if (!in_range_nodes<D>(If, Lf.n)) continue;
+      uint8_t M = nflag<D>(T, l, If);
+      bool cond = (l1 == 0 && f_act(M)) || f_T(M) || f_ghost(M);
+      if (!cond) continue;
+      int64_t qf = lidx<D>(Lf, If);
+      if (qf < 0) continue;
+      float4 val = Lf.grid[qf];
+      float wgt = (l1 == 0) ? 1.f : (l1 == 1 ? 0.5f : (l1 == 2 ? 0.25f : 0.125f));
+      acc.x += val.x * wgt; acc.y += val.y * wgt; acc.z += val.z * wgt; acc.w += val.w * wgt;
+      any = true;
+    }
+    if (!any) continue;
+    float4 g = Lc.grid[q];
+    g.x += acc.x; g.y += acc.y; g.z += acc.z; g.w += acc.w;
+    Lc.grid[q] = g;
+  }
+}
+
+// adaptive_gimp.py:277-287 on every level's list (+ sticky walls / slab collider)
+template <int D>
+__global__ void k_gridop(Topo T, Physics ph) {
+  int tot[MAXL];
+  int total = 0;
+  for (int l = 0; l < T.L; ++l) { tot[l] = T.scount[l] * Pow4<D>::v; total += tot[l]; }
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    int l = 0, r = t;
+    while (r >= tot[l]) { r -= tot[l]; ++l; }
+    const LevelDesc& Ll = T.lv[l];
+    int I[3];
+    uint32_t q = list_node<D>(Ll, r / Pow4<D>::v, r % Pow4<D>::v, I);
+    if (!in_range_nodes<D>(I, Ll.n)) continue;
+    uint8_t M = nflag<D>(T, l, I);
+    if (l >= 1) {
+      bool even = true;
+#pragma unroll
+      for (int a = 0; a < D; ++a) even = even && ((I[a] & 1) == 0);
+      if (f_T(M) || f_ghost(M) || even) continue;  // overwritten by prolongation
+    }
+    float4 g = Ll.grid[q];
+    if (!(g.w > 0.f && f_act(M))) continue;
+    float v[3] = {g.x, g.y, g.z};
+#pragma unroll
+    for (int a = 0; a < D; ++a) { v[a] /= g.w; v[a] += ph.gravity[a] * ph.dt; }
+    bool wall_any = false;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      int f = I[a] << Ll.shift;
+      bool wl = f < ph.boundary_gap || f > T.finest_size - ph.boundary_gap;
+      wall_any = wall_any || wl;
+      if (wl && ph.wall_mode == 0) v[a] = 0.f;
+    }
+    if (ph.wall_mode == 1 && wall_any)
+#pragma unroll
+      for (int a = 0; a < D; ++a) v[a] = 0.f;
+    if (ph.slab_enable) {
+      double y = (double)I[ph.slab_axis] * (double)Ll.dx;
+      if (y >= (double)ph.slab_lo && y <= (double)ph.slab_hi)
+#pragma unroll
+        for (int a = 0; a < D; ++a) v[a] = 0.f;
+    }
+    g.x = v[0]; g.y = v[1];
+    if (D == 3) g.z = v[2];
+    Ll.grid[q] = g;
+  }
+}
+
+// adaptive_gimp.py:303-321 as a gather onto each level-l node J that is T, ghost or
+// all-even: J = 2I + d over valid coarse nodes I, weight 2^-|d|_1 (only d = 0 unless T/G).
+template <int D>
+__global__ void k_prolong(Topo T, int l) {
+  const LevelDesc& Lf = T.lv[l];
+  const LevelDesc& Lc = T.lv[l - 1];
+  int total = T.scount[l] * Pow4<D>::v;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    int J[3];
+    uint32_t q = list_node<D>(Lf, t / Pow4<D>::v, t % Pow4<D>::v, J);
+    if (!in_range_nodes<D>(J, Lf.n)) continue;
+    uint8_t M = nflag<D>(T, l, J);
+    bool isT = f_T(M), isG = f_ghost(M), isA = f_act(M);
+    bool even = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) even = even && ((J[a] & 1) == 0);
+    if (!(isT || isG || even)) continue;
+    float acc[3] = {0.f, 0.f, 0.f};
+    if (isT || isG || isA) {
+      int nodd = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) nodd += (J[a] & 1);
+      int combos = 1 << nodd;
+      for (int u = 0; u < combos; ++u) {
+        int Ic[3], bit = 0, l1 = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          int d = 0;
+          if (J[a] & 1) { d = ((u >> bit) & 1) ? 1 : -1; ++bit; ++l1; }
+          Ic[a] = (J[a] - d) >> 1;
+        }
+        if (l1 > 0 && !(isT || isG)) continue;
+        if (!in_range_nodes<D>(Ic, Lc.n)) continue;
+        if (!f_valid(nflag<D>(T, l - 1, Ic))) continue;
+        int64_t qc = lidx<D>(Lc, Ic);
+        if (qc < 0) continue;
+        float4 vc = Lc.grid[qc];
+        float wgt = (l1 == 0) ? 1.f : (l1 == 1 ? 0.5f : (l1 == 2 ? 0.25f : 0.125f));
+        acc[0] += vc.x * wgt; acc[1] += vc.y * wgt; acc[2] += vc.z * wgt;
+      }
+    }
+    float4 g = Lf.grid[q];
+    g.x = acc[0]; g.y = acc[1];
+    if (D == 3) g.z = acc[2];
+    Lf.grid[q] = g;
+  }
+}
+
+// adaptive_gimp.py:252-275 (+ plasticity and the next substep's stress); reads slot
+// perm[pos] of the current buffer, writes slot pos of the next buffer (re-sorting for free).
+template <int D>
+__global__ void __launch_bounds__(256) k_g2p(Topo T, PBuf Pin, PBuf Pout, const int* perm, const int* gb,
+                                             int* flv, int n, Physics ph, unsigned long long* cnt) {
+  using PL = PLayout<D>;
+  int pos = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pos >= n) return;
+  int k = perm[pos];
+  int g = gb[k];
+  if (g < 0) {  // frozen (fl_p = -1, adaptive_gimp.py:211-216): carried over unchanged
+    for (int c = 0; c < PL::NF; ++c) Pout.f[(size_t)c * Pout.stride + pos] = Pin.f[(size_t)c * Pin.stride + k];
+    Pout.g[pos] = Pin.g[k];
+    Pout.id[pos] = Pin.id[k];
+    flv[pos] = -1;
+    return;
+  }
+  int l = level_of_gbid(T, g);
+  const LevelDesc& Ll = T.lv[l];
+  float x[3], w[3][3], dw[3][3];
+  int J[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    x[a] = Pin.f[(size_t)(PL::X + a) * Pin.stride + k];
+    float qx = scaled(x[a], Ll, T.pow2);
+    int b = (int)(qx + 0.5f) - 1;
+    bspline(qx - (float)(b + 1), w[a], dw[a]);
+    J[a] = b + PAD;
+  }
+  float nv[3] = {0.f, 0.f, 0.f}, G[3][3] = {};
+  if constexpr (D == 2) {
+#pragma unroll
+    for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+      for (int i1 = 0; i1 < 3; ++i1) {
+        int Jn[2] = {J[0] + i0, J[1] + i1};
+        float4 gv = Ll.grid[bidx<2>(Ll.NB, Jn)];
+        float wt = w[0][i0] * w[1][i1];
+        float g0 = dw[0][i0] * w[1][i1], g1 = w[0][i0] * dw[1][i1];
+        nv[0] += gv.x * wt; nv[1] += gv.y * wt;
+        G[0][0] += gv.x * g0; G[0][1] += gv.x * g1;
+        G[1][0] += gv.y * g0; G[1][1] += gv.y * g1;
+      }
+  } else {
+#pragma unroll
+    for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+      for (int i1 = 0; i1 < 3; ++i1)
+#pragma unroll
+        for (int i2 = 0; i2 < 3; ++i2) {
+          int Jn[3] = {J[0] + i0, J[1] + i1, J[2] + i2};
+          float4 gv = Ll.grid[bidx<3>(Ll.NB, Jn)];
+          float w01 = w[0][i0] * w[1][i1];
+          float wt = w01 * w[2][i2];
+          float g0 = dw[0][i0] * w[1][i1] * w[2][i2];
+          float g1 = w[0][i0] * dw[1][i1] * w[2][i2];
+          float g2 = w01 * dw[2][i2];
+          float vv[3] = {gv.x, gv.y, gv.z};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            nv[a] += vv[a] * wt;
+            G[a][0] += vv[a] * g0; G[a][1] += vv[a] * g1; G[a][2] += vv[a] * g2;
+          }
+        }
+  }
+  float sc = ph.dt / Ll.dx;  // dt (0.5/r)^d / dx with r = 0.5
+  float Fo[3][3];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Fo[i][j] = Pin.f[(size_t)(PL::F + i * D + j) * Pin.stride + k];
+  double Fd[D][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      float acc = 0.f;
+#pragma unroll
+      for (int q = 0; q < D; ++q) acc += ((i == q ? 1.f : 0.f) + sc * G[i][q]) * Fo[q][j];
+      Fd[i][j] = (double)acc;
+    }
+  double Jp = (double)Pin.f[(size_t)PL::JP * Pin.stride + k];
+  double tau[D][D];
+  constitutive<D>(Fd, Jp, ph, tau, true);
+  bool finite = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float xn = x[a] + ph.dt * nv[a];
+    finite = finite && isfinite(xn) && isfinite(nv[a]);
+    Pout.f[(size_t)(PL::X + a) * Pout.stride + pos] = xn;
+    Pout.f[(size_t)(PL::V + a) * Pout.stride + pos] = nv[a];
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      Pout.f[(size_t)(PL::C + i * D + j) * Pout.stride + pos] = G[i][j];
+      Pout.f[(size_t)(PL::F + i * D + j) * Pout.stride + pos] = (float)Fd[i][j];
+    }
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i; j < D; ++j) Pout.f[(size_t)(PL::TAU + tau_idx<D>(i, j)) * Pout.stride + pos] = (float)tau[i][j];
+  Pout.f[(size_t)PL::M * Pout.stride + pos] = Pin.f[(size_t)PL::M * Pin.stride + k];
+  Pout.f[(size_t)PL::JP * Pout.stride + pos] = (float)Jp;
+  Pout.g[pos] = Pin.g[k];
+  Pout.id[pos] = Pin.id[k];
+  flv[pos] = l;
+  if (!finite) count(cnt, CNT_NAN);
+}
+
+// Kirchhoff stress of the stored F (no plastic projection): after the caller sets state
+template <int D>
+__global__ void k_stress(PBuf P, int n, Physics ph) {
+  using PL = PLayout<D>;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double Fd[D][D], tau[D][D];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Fd[i][j] = (double)P.f[(size_t)(PL::F + i * D + j) * P.stride + k];
+  double Jp = (double)P.f[(size_t)PL::JP * P.stride + k];
+  constitutive<D>(Fd, Jp, ph, tau, false);
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i; j < D; ++j) P.f[(size_t)(PL::TAU + tau_idx<D>(i, j)) * P.stride + k] = (float)tau[i][j];
+}
+
+// ------------------------------------------------------------------------------ host I/O
+// caller order (AoS, like the reference's to_numpy layout) -> slot k = original index
+template <int D>
+__global__ void k_upload(PBuf P, int n, const float* x, const float* v, const float* F, const float* C,
+                         const float* m, const int* g, const float* Jp, int* flv) {
+  using PL = PLayout<D>;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    P.f[(size_t)(PL::X + a) * P.stride + k] = x[(size_t)k * D + a];
+    P.f[(size_t)(PL::V + a) * P.stride + k] = v[(size_t)k * D + a];
+  }
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) {
+    P.f[(size_t)(PL::F + q) * P.stride + k] = F[(size_t)k * D * D + q];
+    P.f[(size_t)(PL::C + q) * P.stride + k] = C[(size_t)k * D * D + q];
+  }
+  P.f[(size_t)PL::M * P.stride + k] = m[k];
+  P.f[(size_t)PL::JP * P.stride + k] = Jp ? Jp[k] : 1.0f;
+  P.g[k] = g[k];
+  P.id[k] = k;
+  flv[k] = 0;  // Taichi fields start zeroed (adaptive_gimp.py:62)
+}
+
+// slot order -> caller order (AoS)
+template <int D>
+__global__ void k_download(PBuf P, int n, const int* flv, float* x, float* v, float* F, float* C, float* Jp,
+                           int* fl, int* g) {
+  using PL = PLayout<D>;
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int o = P.id[k];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    x[(size_t)o * D + a] = P.f[(size_t)(PL::X + a) * P.stride + k];
+    v[(size_t)o * D + a] = P.f[(size_t)(PL::V + a) * P.stride + k];
+  }
+#pragma unroll
+  for (int q = 0; q < D * D; ++q) {
+    F[(size_t)o * D * D + q] = P.f[(size_t)(PL::F + q) * P.stride + k];
+    C[(size_t)o * D * D + q] = P.f[(size_t)(PL::C + q) * P.stride + k];
+  }
+  Jp[o] = P.f[(size_t)PL::JP * P.stride + k];
+  fl[o] = flv[k];
+  g[o] = P.g[k];
+}
+
+// dense readback of one level over [0, e)^d; culled cells (A|C kept for speed) read as the
+// reference's 0 (or GHOST when a ghost mark landed on them later)
+template <int D>
+__global__ void k_dense_level(Topo T, int l, int* cell, float* m, float* v) {
+  const LevelDesc& Ll = T.lv[l];
+  size_t tot = 1;
+  for (int a = 0; a < D; ++a) tot *= (size_t)Ll.e;
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  int I[3];
+  size_t r = t;
+  for (int a = D - 1; a >= 0; --a) { I[a] = (int)(r % Ll.e); r /= Ll.e; }
+  int64_t q = lidx<D>(Ll, I);
+  uint8_t c = Ll.cell[q];
+  if (c & CULLING) c &= GHOST;
+  cell[t] = c;
+  float4 g = Ll.grid[q];
+  m[t] = g.w;
+  v[t * D + 0] = g.x;
+  v[t * D + 1] = g.y;
+  if (D == 3) v[t * D + 2] = g.z;
+}
+
+template <int D>
+__global__ void k_dense_node(Topo T, int* out) {
+  const LevelDesc& Ll = T.lv[T.L - 1];
+  size_t tot = 1;
+  for (int a = 0; a < D; ++a) tot *= (size_t)Ll.e;
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= tot) return;
+  int I[3];
+  size_t r = t;
+  for (int a = D - 1; a >= 0; --a) { I[a] = (int)(r % Ll.e); r /= Ll.e; }
+  out[t] = T.node[lidx<D>(Ll, I)];
+}
+
+// ------------------------------------------------------------------------------ launchers
+static inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+#define AGIMP_DIM(D, ...) \
+  do { if ((D) == 2) { constexpr int DD = 2; __VA_ARGS__; } else { constexpr int DD = 3; __VA_ARGS__; } } while (0)
+
+size_t cells_total(const Topo& T, int lmin, int lmax) {
+  size_t s = 0;
+  int p4 = T.dim == 2 ? 16 : 64;
+  for (int l = lmin; l <= lmax; ++l) s += (size_t)T.lv[l].nblocks * p4;
+  return s;
+}
+
+void launch_fill_u32(hipStream_t s, void* p, uint32_t v, size_t n) {
+  if (n == 0) return;
+  unsigned g = (unsigned)std::min<size_t>(blocks_for(n, 256), 65535u * 4);
+  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, (uint32_t*)p, v, n);
+}
+
+void launch_copy_u32(hipStream_t s, void* dst, const void* src, size_t n) {
+  if (n == 0) return;
+  unsigned g = (unsigned)std::min<size_t>(blocks_for(n, 256), 65535u * 4);
+  hipLaunchKernelGGL(k_copy_u32, dim3(g), dim3(256), 0, s, (uint32_t*)dst, (const uint32_t*)src, n);
+}
+
+void launch_clear_sgrid(hipStream_t s, const Topo& T, int grid_blocks) {
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_clear_sgrid<DD>, dim3(grid_blocks), dim3(256), 0, s, T));
+}
+
+void launch_refine(hipStream_t s, const Topo& T, const PBuf& P, int n, int static_mode, const int8_t* ret,
+                   const int8_t* act, unsigned long long* cnt) {
+  unsigned pb = blocks_for(n, 256);
+  AGIMP_DIM(T.dim, {
+    if (static_mode) {
+      hipLaunchKernelGGL(k_activate_static<DD>, dim3(pb), dim3(256), 0, s, T, P, n, ret, act, cnt);
+    } else {
+      hipLaunchKernelGGL(k_activate_dynamic<DD>, dim3(pb), dim3(256), 0, s, T, P, n, cnt);
+      if (T.L > 1) {
+        size_t tm = cells_total(T, 1, T.L - 1);
+        hipLaunchKernelGGL(k_mark_hier<DD>, dim3(blocks_for(tm, 256)), dim3(256), 0, s, T, tm, cnt);
+        size_t ts = cells_total(T, 0, T.L - 2);
+        hipLaunchKernelGGL(k_split<DD>, dim3(blocks_for(ts, 256)), dim3(256), 0, s, T, ts, cnt);
+      }
+    }
+    size_t ta = cells_total(T, 0, T.L - 1);
+    hipLaunchKernelGGL(k_node_A<DD>, dim3(blocks_for(ta, 256)), dim3(256), 0, s, T, ta, cnt);
+    hipLaunchKernelGGL(k_mark_T<DD>, dim3(blocks_for(ta, 256)), dim3(256), 0, s, T, ta, cnt);
+    hipLaunchKernelGGL(k_mark_G<DD>, dim3(blocks_for(ta, 256)), dim3(256), 0, s, T, ta, cnt);
+  });
+}
+
+void launch_bucket(hipStream_t s, const Topo& T, const PBuf& P, int n, SortBufs& S, int CH, unsigned long long* cnt) {
+  unsigned pb = blocks_for(n, 256);
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_levelmap<DD>, dim3(pb), dim3(256), 0, s, T, P, n, S.gb, S.rk, S.bcnt,
+                                      S.frozen, cnt));
+  int nb = T.total_blocks;
+  int ntiles = (nb + SCAN_TILE - 1) / SCAN_TILE;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(SCAN_T), 0, s, S.bcnt, nb, CH, S.tiles);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_T), 0, s, S.tiles, ntiles, S.totals);
+  hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(SCAN_T), 0, s, S.bcnt, nb, CH, S.tiles, S.boff, S.choff);
+  hipLaunchKernelGGL(k_scatter, dim3(pb), dim3(256), 0, s, n, S.gb, S.rk, S.boff, S.totals, S.perm, cnt);
+  AGIMP_DIM(T.dim, {
+    hipLaunchKernelGGL(k_blocks<DD>, dim3(blocks_for(nb, 256)), dim3(256), 0, s, T, S.bcnt, S.boff, S.choff,
+                       S.chunks, CH);
+    hipLaunchKernelGGL(k_scompact<DD>, dim3(blocks_for(nb, 256)), dim3(256), 0, s, T);
+  });
+}
+
+void launch_physics(hipStream_t s, const Topo& T, const PBuf& Pin, const PBuf& Pout, int n, SortBufs& S,
+                    const Physics& ph, int p2g_grid, int grid_grid, unsigned long long* cnt) {
+  AGIMP_DIM(T.dim, {
+    hipLaunchKernelGGL(k_p2g<DD>, dim3(p2g_grid), dim3(256), 0, s, T, Pin, S.perm, S.chunks, S.totals, ph);
+    for (int l = T.L - 1; l >= 1; --l)
+      hipLaunchKernelGGL(k_restrict<DD>, dim3(grid_grid), dim3(256), 0, s, T, l);
+    hipLaunchKernelGGL(k_gridop<DD>, dim3(grid_grid), dim3(256), 0, s, T, ph);
+    for (int l = 1; l < T.L; ++l) hipLaunchKernelGGL(k_prolong<DD>, dim3(grid_grid), dim3(256), 0, s, T, l);
+    hipLaunchKernelGGL(k_g2p<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, Pin, Pout, S.perm, S.gb, S.flv, n,
+                       ph, cnt);
+  });
+}
+
+void launch_stress(hipStream_t s, int dim, const PBuf& P, int n, const Physics& ph) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_stress<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, n, ph));
+}
+
+void launch_upload(hipStream_t s, int dim, const PBuf& P, int n, const float* x, const float* v, const float* F,
+                   const float* C, const float* m, const int* g, const float* Jp, int* flv) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_upload<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, n, x, v, F, C, m, g,
+                                    Jp, flv));
+}
+
+void launch_download(hipStream_t s, int dim, const PBuf& P, int n, const int* flv, float* x, float* v, float* F,
+                     float* C, float* Jp, int* fl, int* g) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_download<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, n, flv, x, v, F,
+                                    C, Jp, fl, g));
+}
+
+void launch_dense_level(hipStream_t s, const Topo& T, int l, int* cell, float* m, float* v) {
+  size_t tot = 1;
+  for (int a = 0; a < T.dim; ++a) tot *= (size_t)T.lv[l].e;
+  AGIMP_DIM(T.dim,
+            hipLaunchKernelGGL(k_dense_level<DD>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, T, l, cell, m, v));
+}
+
+void launch_dense_node(hipStream_t s, const Topo& T, int* out) {
+  size_t tot = 1;
+  for (int a = 0; a < T.dim; ++a) tot *= (size_t)T.lv[T.L - 1].e;
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_dense_node<DD>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, T, out));
+}
+
+}  // namespace agimp

@@ -1,0 +1,296 @@
+"""``AdaptiveGIMP``: drop-in for the simulator class of g1n0st/adaptive-gimp.
+
+Same constructor (``adaptive_gimp.py:6-11``), ``substep(dt0)`` (``:323``), the same particle
+fields (``x_p v_p F_p C_p m_p c_p g_p fl_p``, ``:55-62``) and per-level grid fields
+(``cell_mask ad_grid_v ad_grid_m``, ``:39-42``; ``node_mask``, ``:23-26``), plus an explicit
+``refine()``.  Fields behave like Taichi fields from Python scope: ``to_numpy()``,
+``from_numpy()``, indexing.  All computation runs in libagimp.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .scenes import extents, static_level_maps
+
+UNACTIVATED, ACTIVATED, GHOST, T_JUNCTION, CULLING = 0, 1, 2, 4, 8  # utils.py:5-9
+
+
+class ParticleField:
+    """Host view of one particle field, synchronised with the device on access."""
+
+    def __init__(self, sim, name):
+        self._sim, self._name = sim, name
+
+    @property
+    def shape(self):
+        return self._sim._host[self._name].shape
+
+    def to_numpy(self):
+        self._sim._pull()
+        return self._sim._host[self._name].copy()
+
+    def from_numpy(self, arr):
+        self._sim._pull()
+        h = self._sim._host[self._name]
+        h[...] = np.asarray(arr, h.dtype).reshape(h.shape)
+        self._sim._host_dirty = True
+
+    def fill(self, value):
+        self._sim._pull()
+        self._sim._host[self._name][...] = value
+        self._sim._host_dirty = True
+
+    def __getitem__(self, i):
+        self._sim._pull()
+        return self._sim._host[self._name][i]
+
+    def __setitem__(self, i, value):
+        self._sim._pull()
+        self._sim._host[self._name][i] = value
+        self._sim._host_dirty = True
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.to_numpy()
+        return a if dtype is None else a.astype(dtype)
+
+
+class GridField:
+    """Read-only dense view of one level's cell_mask / ad_grid_m / ad_grid_v."""
+
+    def __init__(self, sim, level, kind):
+        self._sim, self._l, self._kind = sim, level, kind
+
+    @property
+    def shape(self):
+        e = self._sim.extents[self._l]
+        shp = (e,) * self._sim.dim
+        return shp + ((self._sim.dim,) if self._kind == "v" else ())
+
+    def to_numpy(self):
+        return self._sim._read_level(self._l)[self._kind]
+
+    def __getitem__(self, i):
+        return self.to_numpy()[i]
+
+
+class NodeMaskField:
+    def __init__(self, sim):
+        self._sim = sim
+
+    @property
+    def shape(self):
+        return (self._sim.extents[-1],) * self._sim.dim
+
+    def to_numpy(self):
+        return self._sim._read_node_mask()
+
+    def __getitem__(self, i):
+        return self.to_numpy()[i]
+
+
+class AdaptiveGIMP:
+    """GPU adaptive GIMP simulator (reference: ``adaptive_gimp.py:4-352``).
+
+    Keyword parameters extend the reference's hard-coded constants; every default is the
+    reference's value: ``E=5e3, nu=0.2`` (``:52``), ``gravity`` -100 on axis 1 (``:50``),
+    ``p_mass=1`` (``:49``), particle volume 1 (implicit in the stress scale, ``:240``),
+    ``boundary_gap=4`` (``:18``), slip walls (``:284-287``), no plasticity.
+    """
+
+    def __init__(self, dim, level, coarsest_size, n_particles, particle_initializer, grid_initializer=None, *,
+                 E=5e3, nu=0.2, gravity=None, p_mass=1.0, p_vol=1.0, boundary_gap=4, wall_mode="slip",
+                 plastic=False, theta_c=2.5e-2, theta_s=7.5e-3, hardening=0.0, slab=None,
+                 particles_per_chunk=256, device=0):
+        if dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        self.dim = dim
+        self.level = level
+        self.coarsest_size = coarsest_size
+        self.coarsest_dx = 1 / coarsest_size                      # adaptive_gimp.py:16
+        self.finest_size = coarsest_size * (2 ** (level - 1))     # :17
+        self.boundary_gap = boundary_gap                          # :18
+        self.leaf_size = 4                                        # :28
+        self.radius = 0.5                                         # :48
+        self.p_mass = p_mass                                      # :49
+        g = [-100.0 if a == 1 else 0.0 for a in range(dim)] if gravity is None else list(gravity)
+        self.gravity = np.asarray(g, np.float64)                  # :50
+        self.n_particles = int(n_particles)                       # :51
+        self.E, self.nu = E, nu
+        self.mu = E / (2 * (1 + nu))                              # :53
+        self.la = E * nu / ((1 + nu) * (1 - 2 * nu))
+        self.extents = extents(level, coarsest_size)
+
+        self._L = _lib.load()
+        cfg = _lib.AgimpConfig()
+        _lib.check(self._L.agimp_config_init(ctypes.byref(cfg), dim, level, coarsest_size, self.n_particles),
+                   what="agimp_config_init")
+        cfg.boundary_gap = boundary_gap
+        cfg.E, cfg.nu = E, nu
+        for a in range(3):
+            cfg.gravity[a] = float(g[a]) if a < len(g) else 0.0
+        cfg.p_vol = p_vol
+        cfg.wall_mode = {"slip": 0, "sticky": 1}.get(wall_mode, wall_mode) if isinstance(wall_mode, str) \
+            else int(wall_mode)
+        cfg.plastic = int(bool(plastic))
+        cfg.theta_c, cfg.theta_s, cfg.hardening = theta_c, theta_s, hardening
+        if slab is not None:
+            cfg.slab_enable, cfg.slab_axis, cfg.slab_lo, cfg.slab_hi = 1, int(slab[0]), float(slab[1]), float(slab[2])
+        cfg.particles_per_chunk = int(particles_per_chunk)
+        cfg.device = int(device)
+        self._cfg = cfg
+        h = ctypes.c_void_p()
+        _lib.check(self._L.agimp_create(ctypes.byref(cfg), ctypes.byref(h)), what="agimp_create")
+        self._h = h
+
+        n, d = self.n_particles, dim
+        self._host = {
+            "x_p": np.zeros((n, d), np.float32), "v_p": np.zeros((n, d), np.float32),
+            "F_p": np.zeros((n, d, d), np.float32), "C_p": np.zeros((n, d, d), np.float32),
+            "m_p": np.zeros(n, np.float32), "c_p": np.zeros((n, 3), np.float32),
+            "g_p": np.zeros(n, np.int32), "fl_p": np.zeros(n, np.int32), "Jp_p": np.ones(n, np.float32),
+        }
+        self._host_dirty = True
+        self._device_newer = False
+        self._level_cache = None
+        self._node_cache = None
+        for name in self._host:
+            setattr(self, name, ParticleField(self, name))
+        self.cell_mask = [GridField(self, l, "cell") for l in range(level)]
+        self.ad_grid_m = [GridField(self, l, "m") for l in range(level)]
+        self.ad_grid_v = [GridField(self, l, "v") for l in range(level)]
+        self.node_mask = NodeMaskField(self)
+
+        particle_initializer(self)                                # :64
+        self.grid_initializer = grid_initializer                  # :65
+        self.static_adaptivity = grid_initializer is not None     # :66
+        if self.static_adaptivity:
+            ret, act = static_level_maps(grid_initializer, dim, level, coarsest_size)
+            self._static_maps = (np.ascontiguousarray(ret), np.ascontiguousarray(act))
+            _lib.check(self._L.agimp_set_static_levels(self._h, _lib.ptr(self._static_maps[0], ctypes.c_int8),
+                                                       _lib.ptr(self._static_maps[1], ctypes.c_int8)),
+                       self._h, "agimp_set_static_levels")
+        self._push()
+
+    # ------------------------------------------------------------------ lifecycle
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.agimp_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ host <-> device
+    def _push(self):
+        if not self._host_dirty:
+            return
+        H = self._host
+        arrs = [np.ascontiguousarray(H[k], np.float32) for k in ("x_p", "v_p", "F_p", "C_p", "m_p")]
+        g = np.ascontiguousarray(H["g_p"], np.int32)
+        jp = np.ascontiguousarray(H["Jp_p"], np.float32)
+        f = ctypes.c_float
+        _lib.check(self._L.agimp_set_particles(self._h, *[_lib.ptr(a, f) for a in arrs], _lib.ptr(g, ctypes.c_int32),
+                                               _lib.ptr(jp, f)), self._h, "agimp_set_particles")
+        self._host_dirty = False
+
+    def _pull(self):
+        if not self._device_newer:
+            return
+        H = self._host
+        f, i = ctypes.c_float, ctypes.c_int32
+        _lib.check(self._L.agimp_get_particles(self._h, _lib.ptr(H["x_p"], f), _lib.ptr(H["v_p"], f),
+                                               _lib.ptr(H["F_p"], f), _lib.ptr(H["C_p"], f),
+                                               _lib.ptr(H["Jp_p"], f), _lib.ptr(H["fl_p"], i),
+                                               _lib.ptr(H["g_p"], i)), self._h, "agimp_get_particles")
+        self._device_newer = False
+
+    def _invalidate(self):
+        self._device_newer = True
+        self._level_cache = None
+        self._node_cache = None
+
+    def _read_level(self, l):
+        if self._level_cache is None:
+            self._level_cache = {}
+        if l not in self._level_cache:
+            e, d = self.extents[l], self.dim
+            cell = np.empty((e,) * d, np.int32)
+            m = np.empty((e,) * d, np.float32)
+            v = np.empty((e,) * d + (d,), np.float32)
+            _lib.check(self._L.agimp_get_level(self._h, l, _lib.ptr(cell, ctypes.c_int32), _lib.ptr(m, ctypes.c_float),
+                                               _lib.ptr(v, ctypes.c_float)), self._h, "agimp_get_level")
+            self._level_cache[l] = {"cell": cell, "m": m, "v": v}
+        return self._level_cache[l]
+
+    def _read_node_mask(self):
+        if self._node_cache is None:
+            out = np.empty((self.extents[-1],) * self.dim, np.int32)
+            _lib.check(self._L.agimp_get_node_mask(self._h, _lib.ptr(out, ctypes.c_int32)), self._h,
+                       "agimp_get_node_mask")
+            self._node_cache = out
+        return self._node_cache
+
+    # ------------------------------------------------------------------ simulation
+    def refine(self):
+        """Rebuild the octree from the current particles (adaptive_gimp.py:324-335)."""
+        self._push()
+        _lib.check(self._L.agimp_refine(self._h), self._h, "agimp_refine")
+        self._level_cache = None
+        self._node_cache = None
+
+    def substep(self, dt0):
+        """adaptive_gimp.py:323-352."""
+        self._push()
+        _lib.check(self._L.agimp_substep(self._h, float(dt0)), self._h, "agimp_substep")
+        self._invalidate()
+
+    def substep_fixed_topology(self, dt0):
+        """One substep on the current octree (no rebuild)."""
+        self._push()
+        _lib.check(self._L.agimp_substep_fixed_topology(self._h, float(dt0)), self._h, "agimp_substep_fixed_topology")
+        self._invalidate()
+
+    def advance(self, dt0, n):
+        self._push()
+        _lib.check(self._L.agimp_advance(self._h, float(dt0), int(n)), self._h, "agimp_advance")
+        self._invalidate()
+
+    def sync(self):
+        _lib.check(self._L.agimp_sync(self._h), self._h, "agimp_sync")
+
+    def counters(self):
+        out = np.zeros(8, np.int64)
+        _lib.check(self._L.agimp_get_counters(self._h, _lib.ptr(out, ctypes.c_int64), 8), self._h, "get_counters")
+        return {"oob": int(out[0]), "frozen": int(out[1]), "active": int(out[2]), "chunks": int(out[3]),
+                "nonfinite": int(out[4])}
+
+    # ------------------------------------------------------------------ reference helpers
+    def activate_cell(self, l, I):
+        raise RuntimeError("activate_cell is only meaningful inside a grid_initializer call")
+
+    def _map(self, l, I):  # adaptive_gimp.py:126-128
+        return np.asarray(I) * (2 ** (self.level - 1 - l))
+
+    def _nm(self, l, I):
+        M = self._read_node_mask()
+        f = self._map(l, I)
+        return M[tuple(np.moveaxis(np.asarray(f), -1, 0))]
+
+    def is_valid(self, l, I):        # :130-131
+        return self._nm(l, I) > 0
+
+    def is_activated(self, l, I):    # :133-134
+        return (self._nm(l, I) & (ACTIVATED | T_JUNCTION)) == ACTIVATED
+
+    def is_T_junction(self, l, I):   # :136-137
+        return (self._nm(l, I) & T_JUNCTION) == T_JUNCTION
+
+    def is_ghost(self, l, I):        # :139-140
+        return (self._nm(l, I) & (ACTIVATED | GHOST | T_JUNCTION)) == GHOST
