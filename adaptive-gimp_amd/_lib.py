@@ -11,6 +11,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libagimp.so")
 
+PHASES = ["refine", "bucket", "p2g", "grid", "g2p"]  # AGIMP_PHASE_* order
+
 AGIMP_OK, AGIMP_EINVAL, AGIMP_EHIP, AGIMP_ESTATE, AGIMP_ENOMEM = 0, -1, -2, -3, -4
 
 
@@ -35,7 +37,7 @@ EXPORTS = [
     "agimp_config_init", "agimp_create", "agimp_destroy", "agimp_set_particles", "agimp_set_static_levels",
     "agimp_refine", "agimp_substep", "agimp_substep_fixed_topology", "agimp_advance", "agimp_get_particles",
     "agimp_get_level", "agimp_get_node_mask", "agimp_get_counters", "agimp_reset_counters",
-    "agimp_level_extent", "agimp_sync", "agimp_last_error", "agimp_host_svd", "agimp_host_constitutive",
+    "agimp_level_extent", "agimp_set_timing", "agimp_get_timings", "agimp_sync", "agimp_last_error", "agimp_host_svd", "agimp_host_constitutive",
 ]
 
 _lib = None
@@ -76,6 +78,8 @@ def load():
         "agimp_get_counters": [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32],
         "agimp_reset_counters": [vp],
         "agimp_level_extent": [vp, ctypes.c_int32],
+        "agimp_set_timing": [vp, ctypes.c_int32],
+        "agimp_get_timings": [vp, dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32],
         "agimp_sync": [vp],
         "agimp_last_error": [vp],
         "agimp_host_svd": [ctypes.c_int32, dp, dp, dp, dp],

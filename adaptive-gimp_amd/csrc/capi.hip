@@ -38,6 +38,13 @@ struct agimp_handle {
   int p2g_grid = 0, grid_grid = 0;
   std::vector<void*> allocs;
   std::string err;
+  // phase timing with HIP events on the handle's stream (agimp_set_timing)
+  int timing = 0;
+  struct Mark { int phase; hipEvent_t a, b; };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> event_pool;
+  double phase_ms[AGIMP_NPHASES] = {};
+  int64_t phase_calls[AGIMP_NPHASES] = {};
 };
 
 namespace {
@@ -94,6 +101,53 @@ int check_ready(agimp_handle* h) {
   return 0;
 }
 
+hipEvent_t take_event(agimp_handle* h) {
+  if (!h->event_pool.empty()) {
+    hipEvent_t e = h->event_pool.back();
+    h->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// begin/end a timed phase: events are recorded only while timing is on
+struct PhaseScope {
+  agimp_handle* h;
+  int phase;
+  hipEvent_t a = nullptr;
+  PhaseScope(agimp_handle* hh, int p) : h(hh), phase(p) {
+    if (h->timing) {
+      a = take_event(h);
+      if (a) (void)hipEventRecord(a, h->stream);
+    }
+  }
+  ~PhaseScope() {
+    if (!a) return;
+    hipEvent_t b = take_event(h);
+    if (!b) return;
+    (void)hipEventRecord(b, h->stream);
+    h->marks.push_back({phase, a, b});
+  }
+};
+
+// fold recorded event pairs into the per-phase totals (synchronises the stream)
+int collect_marks(agimp_handle* h) {
+  if (h->marks.empty()) return 0;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (auto& m : h->marks) {
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, m.a, m.b));
+    h->phase_ms[m.phase] += ms;
+    h->phase_calls[m.phase] += 1;
+    h->event_pool.push_back(m.a);
+    h->event_pool.push_back(m.b);
+  }
+  h->marks.clear();
+  return 0;
+}
+
 size_t flag_words(const Topo& T) {
   size_t flags = 0;
   for (int l = 0; l < T.L; ++l) flags += (size_t)T.lv[l].nblocks;
@@ -109,6 +163,7 @@ void clear_prev(agimp_handle* h) {
 
 int do_refine(agimp_handle* h) {
   Topo& T = h->T;
+  PhaseScope ps(h, AGIMP_PHASE_REFINE);
   clear_prev(h);
   // cell masks of all levels are one allocation, as are the block flags
   launch_fill_u32(h->stream, T.lv[0].cell, 0, cells_total(T, 0, T.L - 1) / 4);
@@ -124,15 +179,33 @@ int do_refine(agimp_handle* h) {
 
 int do_physics(agimp_handle* h, float dt) {
   Topo& T = h->T;
-  clear_prev(h);
-  launch_copy_u32(h->stream, T.lv[0].sflag, h->vflag, flag_words(T));
-  launch_fill_u32(h->stream, h->S.bcnt, 0, (size_t)T.total_blocks);
-  launch_fill_u32(h->stream, h->S.frozen, 0, 1);
-  launch_bucket(h->stream, T, h->pa, h->n, h->S, h->CH, h->counters);
+  {
+    PhaseScope ps(h, AGIMP_PHASE_BUCKET);
+    clear_prev(h);
+    launch_copy_u32(h->stream, T.lv[0].sflag, h->vflag, flag_words(T));
+    launch_fill_u32(h->stream, h->S.bcnt, 0, (size_t)T.total_blocks);
+    launch_fill_u32(h->stream, h->S.frozen, 0, 1);
+    launch_bucket(h->stream, T, h->pa, h->n, h->S, h->CH, h->counters);
+  }
   Physics ph = h->ph;
   ph.dt = dt;
-  launch_physics(h->stream, T, h->pa, h->pb, h->n, h->S, ph, h->p2g_grid, h->grid_grid, h->counters);
+  {
+    PhaseScope ps(h, AGIMP_PHASE_P2G);
+    launch_p2g(h->stream, T, h->pa, h->S, ph, h->p2g_grid);
+  }
+  {
+    PhaseScope ps(h, AGIMP_PHASE_GRID);
+    launch_grid(h->stream, T, ph, h->grid_grid);
+  }
+  {
+    PhaseScope ps(h, AGIMP_PHASE_G2P);
+    launch_g2p(h->stream, T, h->pa, h->pb, h->n, h->S, ph, h->counters);
+  }
   std::swap(h->pa, h->pb);
+  if (h->marks.size() > 4096) {
+    int rc = collect_marks(h);
+    if (rc) return rc;
+  }
   h->have_slists = 1;
   HIPCHK(h, hipGetLastError());
   return 0;
@@ -270,6 +343,8 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
 int agimp_destroy(agimp_handle* h) {
   if (!h) return AGIMP_EINVAL;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
+  for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->st_ret) (void)hipFree(h->st_ret);
   if (h->st_act) (void)hipFree(h->st_act);
@@ -450,6 +525,25 @@ int agimp_reset_counters(agimp_handle* h) {
 int agimp_level_extent(agimp_handle* h, int32_t l) {
   if (!h || l < 0 || l >= h->L) return AGIMP_EINVAL;
   return h->T.lv[l].e;
+}
+
+int agimp_set_timing(agimp_handle* h, int32_t on) {
+  if (!h) return AGIMP_EINVAL;
+  h->timing = on ? 1 : 0;
+  return 0;
+}
+
+int agimp_get_timings(agimp_handle* h, double* ms, int64_t* calls, int32_t n) {
+  if (!h || n < 0) return AGIMP_EINVAL;
+  int rc = collect_marks(h);
+  if (rc) return rc;
+  for (int i = 0; i < n && i < AGIMP_NPHASES; ++i) {
+    if (ms) ms[i] = h->phase_ms[i];
+    if (calls) calls[i] = h->phase_calls[i];
+    h->phase_ms[i] = 0.0;
+    h->phase_calls[i] = 0;
+  }
+  return 0;
 }
 
 int agimp_sync(agimp_handle* h) {

@@ -1150,17 +1150,24 @@ void launch_bucket(hipStream_t s, const Topo& T, const PBuf& P, int n, SortBufs&
   });
 }
 
-void launch_physics(hipStream_t s, const Topo& T, const PBuf& Pin, const PBuf& Pout, int n, SortBufs& S,
-                    const Physics& ph, int p2g_grid, int grid_grid, unsigned long long* cnt) {
+void launch_p2g(hipStream_t s, const Topo& T, const PBuf& Pin, SortBufs& S, const Physics& ph, int p2g_grid) {
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_p2g<DD>, dim3(p2g_grid), dim3(256), 0, s, T, Pin, S.perm, S.chunks, S.totals,
+                                      ph));
+}
+
+void launch_grid(hipStream_t s, const Topo& T, const Physics& ph, int grid_grid) {
   AGIMP_DIM(T.dim, {
-    hipLaunchKernelGGL(k_p2g<DD>, dim3(p2g_grid), dim3(256), 0, s, T, Pin, S.perm, S.chunks, S.totals, ph);
     for (int l = T.L - 1; l >= 1; --l)
       hipLaunchKernelGGL(k_restrict<DD>, dim3(grid_grid), dim3(256), 0, s, T, l);
     hipLaunchKernelGGL(k_gridop<DD>, dim3(grid_grid), dim3(256), 0, s, T, ph);
     for (int l = 1; l < T.L; ++l) hipLaunchKernelGGL(k_prolong<DD>, dim3(grid_grid), dim3(256), 0, s, T, l);
-    hipLaunchKernelGGL(k_g2p<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, Pin, Pout, S.perm, S.gb, S.flv, n,
-                       ph, cnt);
   });
+}
+
+void launch_g2p(hipStream_t s, const Topo& T, const PBuf& Pin, const PBuf& Pout, int n, SortBufs& S,
+                const Physics& ph, unsigned long long* cnt) {
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_g2p<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, Pin, Pout, S.perm,
+                                      S.gb, S.flv, n, ph, cnt));
 }
 
 void launch_stress(hipStream_t s, int dim, const PBuf& P, int n, const Physics& ph) {

@@ -28,8 +28,10 @@ void launch_clear_sgrid(hipStream_t s, const Topo& T, int grid_blocks);
 void launch_refine(hipStream_t s, const Topo& T, const PBuf& P, int n, int static_mode, const int8_t* ret,
                    const int8_t* act, unsigned long long* cnt);
 void launch_bucket(hipStream_t s, const Topo& T, const PBuf& P, int n, SortBufs& S, int CH, unsigned long long* cnt);
-void launch_physics(hipStream_t s, const Topo& T, const PBuf& Pin, const PBuf& Pout, int n, SortBufs& S,
-                    const Physics& ph, int p2g_grid, int grid_grid, unsigned long long* cnt);
+void launch_p2g(hipStream_t s, const Topo& T, const PBuf& Pin, SortBufs& S, const Physics& ph, int p2g_grid);
+void launch_grid(hipStream_t s, const Topo& T, const Physics& ph, int grid_grid);
+void launch_g2p(hipStream_t s, const Topo& T, const PBuf& Pin, const PBuf& Pout, int n, SortBufs& S,
+                const Physics& ph, unsigned long long* cnt);
 void launch_stress(hipStream_t s, int dim, const PBuf& P, int n, const Physics& ph);
 void launch_upload(hipStream_t s, int dim, const PBuf& P, int n, const float* x, const float* v, const float* F,
                    const float* C, const float* m, const int* g, const float* Jp, int* flv);

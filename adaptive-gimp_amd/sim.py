@@ -265,6 +265,18 @@ class AdaptiveGIMP:
     def sync(self):
         _lib.check(self._L.agimp_sync(self._h), self._h, "agimp_sync")
 
+    def set_timing(self, on=True):
+        """Record HIP events around every phase of each substep (on the library's stream)."""
+        _lib.check(self._L.agimp_set_timing(self._h, int(bool(on))), self._h, "agimp_set_timing")
+
+    def timings(self):
+        """{phase: (total_ms, calls)} since the last call (synchronises)."""
+        ms = np.zeros(len(_lib.PHASES))
+        calls = np.zeros(len(_lib.PHASES), np.int64)
+        _lib.check(self._L.agimp_get_timings(self._h, _lib.ptr(ms, ctypes.c_double), _lib.ptr(calls, ctypes.c_int64),
+                                             len(_lib.PHASES)), self._h, "agimp_get_timings")
+        return {p: (float(ms[i]), int(calls[i])) for i, p in enumerate(_lib.PHASES)}
+
     def counters(self):
         out = np.zeros(8, np.int64)
         _lib.check(self._L.agimp_get_counters(self._h, _lib.ptr(out, ctypes.c_int64), 8), self._h, "get_counters")

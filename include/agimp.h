@@ -107,6 +107,19 @@ int agimp_get_node_mask(agimp_handle* h, int32_t* node_mask);
 int agimp_get_counters(agimp_handle* h, int64_t* out, int32_t n);
 int agimp_reset_counters(agimp_handle* h);
 
+/* Phase timing with HIP events recorded on the handle's stream around each phase of every
+ * substep (the Taichi kernel profiler of run_3d.py:12, :43 is the reference's analogue).
+ * agimp_get_timings synchronises, returns total milliseconds and call counts per phase
+ * (AGIMP_PHASE_*) accumulated since the last call, and resets them. */
+#define AGIMP_PHASE_REFINE 0  /* clear + activation + culling + node/T/ghost marks */
+#define AGIMP_PHASE_BUCKET 1  /* level mapping + counting sort + block lists */
+#define AGIMP_PHASE_P2G 2     /* the P2G kernel */
+#define AGIMP_PHASE_GRID 3    /* restriction + grid update + prolongation */
+#define AGIMP_PHASE_G2P 4     /* the G2P kernel (incl. plasticity and stress) */
+#define AGIMP_NPHASES 5
+int agimp_set_timing(agimp_handle* h, int32_t on);
+int agimp_get_timings(agimp_handle* h, double* ms, int64_t* calls, int32_t n);
+
 /* Reference extent e_l of level l. */
 int agimp_level_extent(agimp_handle* h, int32_t l);
 

@@ -808,6 +808,17 @@ struct Sim {
     level_mapping();
   }
 
+  // substep on the current octree (no rebuild): zero the grid values, keep every mask,
+  // redo level mapping (adaptive_gimp.py:206-216) and the physics
+  void substep_fixed_topology(R dt) {
+    for (int l = 0; l < L; ++l) {
+      std::fill(gm[l].begin(), gm[l].end(), R(0));
+      std::fill(gv[l].begin(), gv[l].end(), R(0));
+    }
+    level_mapping();
+    physics(dt);
+  }
+
   // ----- adaptive_gimp.py:339-352 (the physics half of substep)
   void physics(R dt) {
     for (int l = 0; l < L; ++l) p2g(l, dt);
@@ -933,6 +944,14 @@ void orc_physics(void* hv, double dt) {
   DISPATCH(h, {
     using R = typename std::remove_reference<decltype(s->x)>::type::value_type;
     s->physics(R((float)dt));  // dt0 is declared ti.f32 (adaptive_gimp.py:231,253,278)
+  });
+}
+
+void orc_substep_fixed_topology(void* hv, double dt) {
+  Handle* h = (Handle*)hv;
+  DISPATCH(h, {
+    using R = typename std::remove_reference<decltype(s->x)>::type::value_type;
+    s->substep_fixed_topology(R((float)dt));
   });
 }
 

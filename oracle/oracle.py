@@ -63,6 +63,7 @@ def lib():
         L.orc_refine.argtypes = [vp]
         L.orc_physics.argtypes = [vp, ctypes.c_double]
         L.orc_substep.argtypes = [vp, ctypes.c_double]
+        L.orc_substep_fixed_topology.argtypes = [vp, ctypes.c_double]
         L.orc_get_level.argtypes = [vp, ctypes.c_int, ip, dp, dp]
         L.orc_get_node_mask.argtypes = [vp, ip]
         L.orc_get_oob.restype = ctypes.c_int64
@@ -148,6 +149,9 @@ class OracleSim:
 
     def substep(self, dt):
         self._lib.orc_substep(self._h, float(dt))
+
+    def substep_fixed_topology(self, dt):
+        self._lib.orc_substep_fixed_topology(self._h, float(dt))
 
     def get_particles(self):
         n, d = self.n, self.dim

@@ -1,0 +1,277 @@
+"""GPU parity: the HIP path (through the C-ABI, via the AdaptiveGIMP facade) against the
+oracle on the same seeded inputs.
+
+Bars:
+  * integer work (cell_mask, node_mask predicates, fl_p) is bit-exact; node_mask raw bits
+    are compared after dropping GHOST where ACTIVATED or T_JUNCTION is set (the reference's
+    own raw bits there depend on a race, adaptive_gimp.py:174 vs :182-183);
+  * FP32 state: grid values after one substep within REL_GRID of the level's largest
+    magnitude; particle v and C within REL_P of the field's largest magnitude, x within
+    ABS_X, F within ABS_F.  Summation order differs (float atomics, gathers), so FP32
+    rounding, not algorithm, sets these bounds.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL_GRID = 2e-5
+REL_P = 2e-5
+ABS_X = 2e-6
+ABS_F = 2e-5
+
+
+def _init_from(st):
+    def init(sim):
+        sim.x_p.from_numpy(st["x"])
+        sim.v_p.from_numpy(st["v"])
+        sim.F_p.from_numpy(st["F"])
+        sim.C_p.from_numpy(st["C"])
+        sim.m_p.from_numpy(st["m"])
+        sim.g_p.from_numpy(st["g"])
+
+    return init
+
+
+def make_pair(pkg, O, dim, L, c, st, params, gi=None):
+    n = st["x"].shape[0]
+    sim = pkg.AdaptiveGIMP(dim, L, c, n, _init_from(st), gi, **params)
+    oprm = dict(params)
+    if isinstance(oprm.get("wall_mode"), str):
+        oprm["wall_mode"] = {"slip": 0, "sticky": 1}[oprm["wall_mode"]]
+    if "plastic" in oprm:
+        oprm["plastic"] = int(oprm["plastic"])
+    ora = O.OracleSim(dim, L, c, n, fp64=False, **oprm)
+    f32 = lambda a: np.asarray(a, np.float32)
+    ora.set_particles(f32(st["x"]), f32(st["v"]), f32(st["F"]), f32(st["C"]), f32(st["m"]), st["g"])
+    if gi is not None:
+        ret, act = pkg.scenes.static_level_maps(gi, dim, L, c)
+        ora.set_static_map(ret, act)
+    return sim, ora
+
+
+def assert_topology_equal(sim, ora, O, L):
+    for l in range(L):
+        a = sim.cell_mask[l].to_numpy()
+        b = ora.get_level(l)[0]
+        assert np.array_equal(a, b), f"cell_mask[{l}]: {(a != b).sum()} cells differ"
+    a = O.canonical_node_mask(sim.node_mask.to_numpy())
+    b = O.canonical_node_mask(ora.get_node_mask())
+    assert np.array_equal(a, b), f"node_mask: {(a != b).sum()} nodes differ"
+
+
+def assert_grid_close(sim, ora, L):
+    for l in range(L):
+        _, mr, vr = ora.get_level(l)
+        mg, vg = sim.ad_grid_m[l].to_numpy(), sim.ad_grid_v[l].to_numpy()
+        ms, vs = max(np.abs(mr).max(), 1e-30), max(np.abs(vr).max(), 1e-30)
+        assert np.abs(mg - mr).max() <= REL_GRID * ms, f"m[{l}] {np.abs(mg - mr).max() / ms:.2e}"
+        assert np.abs(vg - vr).max() <= REL_GRID * vs, f"v[{l}] {np.abs(vg - vr).max() / vs:.2e}"
+
+
+def particle_errors(sim, ora):
+    r = ora.get_particles()
+    g = {k: getattr(sim, k).to_numpy() for k in ("x_p", "v_p", "F_p", "C_p", "fl_p", "Jp_p")}
+    rel = lambda a, b: np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+    return dict(fl=int((g["fl_p"] != r["fl"]).sum()), x=float(np.abs(g["x_p"] - r["x"]).max()),
+                v=float(rel(g["v_p"], r["v"])), C=float(rel(g["C_p"], r["C"])),
+                F=float(np.abs(g["F_p"] - r["F"]).max()), Jp=float(np.abs(g["Jp_p"] - r["Jp"]).max()))
+
+
+def assert_particles_close(e, x_tol=ABS_X, rel=REL_P, f_tol=ABS_F):
+    assert e["fl"] == 0, e
+    assert e["x"] <= x_tol and e["v"] <= rel and e["C"] <= 10 * rel and e["F"] <= f_tol and e["Jp"] <= f_tol, e
+
+
+class _Rec:
+    """Collects what a reference-style initializer writes."""
+
+    def __init__(self, n):
+        self.n_particles, self.p_mass, self.d = n, 1.0, {}
+
+    def __getattr__(self, k):
+        if k.endswith("_p"):
+            rec = self
+
+            class P:
+                def from_numpy(self, a):
+                    rec.d[k] = np.asarray(a)
+
+            return P()
+        raise AttributeError(k)
+
+
+def ref_state(init, n, dim):
+    r = _Rec(n)
+    init(r)
+    d = r.d
+    return dict(x=d["x_p"], v=d["v_p"], F=d["F_p"], C=np.zeros((n, dim, dim)), m=d["m_p"], g=d["g_p"])
+
+
+SCENES = ["ref2d_dynamic", "ref3d_dynamic", "small3d_plastic", "c1_two_squares"]
+
+
+def scene(pkg, name):
+    S = pkg.scenes
+    if name == "ref2d_dynamic":  # run.py:17-26 case 0
+        return 2, 4, 16, ref_state(S.reference_case0_2d(0), 20000, 2), {}, None, 2e-5
+    if name == "ref3d_dynamic":  # run_3d.py:34
+        return 3, 3, 16, ref_state(S.reference_3d(), 50000, 3), {}, None, 1e-4
+    if name == "small3d_plastic":
+        wl = S.small_3d(seed=1, n=6000, L=3, c=16)
+        return 3, wl.level, wl.coarsest_size, wl.state, wl.params, None, wl.dt
+    if name == "c1_two_squares":
+        wl = S.c1_two_squares()
+        return 2, wl.level, wl.coarsest_size, wl.state, wl.params, None, wl.dt
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_refine_topology_bit_exact(pkg, O, name):
+    dim, L, c, st, prm, gi, dt = scene(pkg, name)
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    sim.refine()
+    ora.refine()
+    assert_topology_equal(sim, ora, O, L)
+    sim.close()
+
+
+@pytest.mark.parametrize("case", [1, 2, 3, 4, 5])
+def test_static_masks_bit_exact_and_one_substep(pkg, O, case):
+    """run.py --case 1..5: static adaptivity with init.py's grid initializers."""
+    st = ref_state(pkg.scenes.reference_case0_2d(case), 20000, 2)
+    gi = pkg.scenes.REFERENCE_GRID_INITIALIZERS[case]
+    sim, ora = make_pair(pkg, O, 2, 4, 16, st, {}, gi)
+    sim.refine()
+    ora.refine()
+    assert_topology_equal(sim, ora, O, 4)
+    sim.substep(2e-5)
+    ora.substep(2e-5)
+    assert_grid_close(sim, ora, 4)
+    assert_particles_close(particle_errors(sim, ora))
+    sim.close()
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_one_substep_grid_and_particles(pkg, O, name):
+    dim, L, c, st, prm, gi, dt = scene(pkg, name)
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    sim.substep(dt)
+    ora.substep(dt)
+    assert_topology_equal(sim, ora, O, L)
+    assert_grid_close(sim, ora, L)
+    assert_particles_close(particle_errors(sim, ora))
+    assert sim.counters()["oob"] == ora.oob() == 0
+    sim.close()
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_many_substeps_track_oracle(pkg, O, name):
+    """10 substeps (stress and affine terms active from step 2), then the topology of both
+    rebuilt from the GPU's particle snapshot must agree bit for bit."""
+    dim, L, c, st, prm, gi, dt = scene(pkg, name)
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    for _ in range(10):
+        sim.substep(dt)
+        ora.substep(dt)
+    e = particle_errors(sim, ora)
+    assert e["fl"] == 0 and e["x"] <= 1e-5 and e["v"] <= 1e-3 and e["F"] <= 1e-4, e
+    # integer parity from identical snapshots
+    snap = dict(x=sim.x_p.to_numpy(), v=sim.v_p.to_numpy(), F=sim.F_p.to_numpy(), C=sim.C_p.to_numpy(),
+                m=st["m"], g=sim.g_p.to_numpy())
+    o2 = O.OracleSim(dim, L, c, snap["x"].shape[0], fp64=False,
+                     **{k: (int(v) if k == "plastic" else v) for k, v in prm.items()})
+    o2.set_particles(snap["x"], snap["v"], snap["F"], snap["C"], np.asarray(snap["m"], np.float32), snap["g"])
+    sim.refine()
+    o2.refine()
+    assert_topology_equal(sim, o2, O, L)
+    sim.close()
+
+
+def test_fixed_topology_substeps(pkg, O):
+    dim, L, c, st, prm, gi, dt = scene(pkg, "small3d_plastic")
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    sim.substep(dt)
+    ora.substep(dt)
+    for _ in range(3):
+        sim.substep_fixed_topology(dt)
+        ora.substep_fixed_topology(dt)
+    assert_grid_close(sim, ora, L)
+    assert_particles_close(particle_errors(sim, ora), x_tol=5e-6, rel=1e-4, f_tol=1e-4)
+    sim.close()
+
+
+def test_frozen_particles_unchanged(pkg, O):
+    """run.py --case != 0 without a grid initializer: g_p = -1 everywhere, nothing is
+    activated, fl_p = -1 and every particle is skipped (adaptive_gimp.py:211-216)."""
+    st = ref_state(pkg.scenes.reference_case0_2d(1), 5000, 2)
+    sim, ora = make_pair(pkg, O, 2, 4, 16, st, {})
+    sim.substep(2e-5)
+    ora.substep(2e-5)
+    np.testing.assert_array_equal(sim.x_p.to_numpy(), np.asarray(st["x"], np.float32))
+    np.testing.assert_array_equal(sim.v_p.to_numpy(), np.asarray(st["v"], np.float32))
+    assert (sim.fl_p.to_numpy() == -1).all()
+    assert sim.counters()["frozen"] == 5000
+    for l in range(4):
+        assert not sim.cell_mask[l].to_numpy().any()
+    sim.close()
+
+
+def test_wall_band_and_negative_base_cells(pkg, O):
+    """Particles inside the wall band, some with base cell -1 at every level."""
+    rng = np.random.default_rng(3)
+    n = 4000
+    x = rng.uniform(0.005, 0.995, (n, 2))
+    st = dict(x=x, v=rng.standard_normal((n, 2)), F=np.tile(np.eye(2), (n, 1, 1)), C=np.zeros((n, 2, 2)),
+              m=np.ones(n), g=rng.integers(0, 3, n).astype(np.int32))
+    sim, ora = make_pair(pkg, O, 2, 3, 16, st, {})
+    sim.substep(1e-5)
+    ora.substep(1e-5)
+    assert_topology_equal(sim, ora, O, 3)
+    assert_grid_close(sim, ora, 3)
+    assert_particles_close(particle_errors(sim, ora))
+    sim.close()
+
+
+def test_single_particle(pkg, O):
+    st = dict(x=np.array([[0.5, 0.5, 0.5]]), v=np.array([[1.0, 0.0, 0.0]]), F=np.eye(3)[None], C=np.zeros((1, 3, 3)),
+              m=np.ones(1), g=np.array([2], np.int32))
+    sim, ora = make_pair(pkg, O, 3, 3, 8, st, {})
+    for _ in range(3):
+        sim.substep(1e-4)
+        ora.substep(1e-4)
+    assert_particles_close(particle_errors(sim, ora))
+    sim.close()
+
+
+def test_headline_scene_full_size_properties(pkg, O):
+    """The 1M-particle 3-level sphere at full size: topology bit-exact vs the oracle after 3
+    substeps (from the GPU's own snapshot), every particle finite and transferred, and mass
+    conserved on canonical + unmarked nodes to FP32 accumulation accuracy."""
+    wl = pkg.scenes.c3_sphere_on_slab()
+    sim = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, wl.n, wl.initializer(), **wl.params)
+    for _ in range(3):
+        sim.substep(wl.dt)
+    cnt = sim.counters()
+    assert cnt["oob"] == 0 and cnt["nonfinite"] == 0 and cnt["frozen"] == 0 and cnt["active"] == 3 * wl.n
+    x = sim.x_p.to_numpy()
+    assert np.isfinite(x).all()
+    nm = sim.node_mask.to_numpy()
+    total = 0.0
+    for l in range(wl.level):
+        m = sim.ad_grid_m[l].to_numpy().astype(np.float64)
+        s = 2 ** (wl.level - 1 - l)
+        nz = np.argwhere(m != 0)
+        f = nz * s
+        M = nm[f[:, 0], f[:, 1], f[:, 2]]
+        act = (M & 5) == 1
+        canon = act if l == 0 else act & np.any(nz % 2 == 1, axis=1)
+        total += m[tuple(nz[canon].T)].sum() + m[tuple(nz[M == 0].T)].sum()
+    assert total == pytest.approx(wl.state["m"].sum(), rel=1e-4)
+    o2 = O.OracleSim(3, wl.level, wl.coarsest_size, wl.n, fp64=False, **wl.params)
+    o2.set_particles(x, sim.v_p.to_numpy(), sim.F_p.to_numpy(), sim.C_p.to_numpy(),
+                     np.asarray(wl.state["m"], np.float32), sim.g_p.to_numpy())
+    sim.refine()
+    o2.refine()
+    assert_topology_equal(sim, o2, O, wl.level)
+    sim.close()
