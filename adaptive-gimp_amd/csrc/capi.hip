@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/agimp.h"
+#include "agimp.h"
 #include "common.hpp"
 #include "launch.hpp"
 #include "svd.hpp"
@@ -247,7 +247,7 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   if (c.level < 1 || c.level > MAXL) return AGIMP_EINVAL;
   if (c.coarsest_size < 1 || c.n_particles < 1 || c.n_particles > (int64_t)1 << 30) return AGIMP_EINVAL;
   if (std::fabs(c.radius - 0.5f) > 0.0f) return AGIMP_EINVAL;  // B-spline form needs r = 0.5
-  if (c.particles_per_chunk < 16 || c.particles_per_chunk > 65536) return AGIMP_EINVAL;
+  if (c.particles_per_chunk < 16 || c.particles_per_chunk > 2048) return AGIMP_EINVAL;  // P2G_MAX_CH
   if (c.slab_enable && (c.slab_axis < 0 || c.slab_axis >= c.dim)) return AGIMP_EINVAL;
   agimp_handle* h = new agimp_handle();
   h->cfg = c;

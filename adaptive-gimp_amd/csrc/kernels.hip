@@ -568,6 +568,48 @@ __global__ void k_scompact(Topo T) {
   if (on) T.lv[l].slist[idx] = b;
 }
 
+// ------------------------------------------------------------------------------ wave reductions
+// DPP moves (GFX9 row_shr / row_bcast); lanes without a source keep `old`
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int src, int old) {
+  return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWMASK, 0xf, false);
+}
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float dpp_f(float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(src), CTRL, ROWMASK, 0xf, false));
+}
+
+// Which DPP steps of a segmented inclusive scan add their source: lanes sorted by key form
+// contiguous segments; a step adds only when the source lane carries the same key.
+struct SegMask {
+  bool s1, s2, s4, s8, b15, b31, tail;
+};
+
+__device__ __forceinline__ SegMask seg_mask(int key) {
+  SegMask m;
+  m.s1 = dpp_i<0x111, 0xf>(key, -1) == key;
+  m.s2 = dpp_i<0x112, 0xf>(key, -1) == key;
+  m.s4 = dpp_i<0x114, 0xf>(key, -1) == key;
+  m.s8 = dpp_i<0x118, 0xf>(key, -1) == key;
+  m.b15 = dpp_i<0x142, 0xa>(key, -1) == key;
+  m.b31 = dpp_i<0x143, 0xc>(key, -1) == key;
+  int nxt = __shfl_down(key, 1);
+  m.tail = (__lane_id() == 63) || (nxt != key);
+  return m;
+}
+
+// Segmented inclusive scan: a segment's last lane ends up holding the segment's sum.
+__device__ __forceinline__ float seg_sum(float v, const SegMask& m) {
+  float t;
+  t = dpp_f<0x111, 0xf>(v); v += m.s1 ? t : 0.f;
+  t = dpp_f<0x112, 0xf>(v); v += m.s2 ? t : 0.f;
+  t = dpp_f<0x114, 0xf>(v); v += m.s4 ? t : 0.f;
+  t = dpp_f<0x118, 0xf>(v); v += m.s8 ? t : 0.f;
+  t = dpp_f<0x142, 0xa>(v); v += m.b15 ? t : 0.f;
+  t = dpp_f<0x143, 0xc>(v); v += m.b31 ? t : 0.f;
+  return v;
+}
+
 // ------------------------------------------------------------------------------ transfers
 // Quadratic B-spline weights and their t-derivatives: for r = 0.5 the GIMP overlap
 // integrals of utils.py:18-42 summed by adaptive_gimp.py:218-228 reduce to these.
@@ -581,58 +623,120 @@ __device__ __forceinline__ void bspline(float t, float* w, float* dw) {
   dw[2] = c;
 }
 
-// adaptive_gimp.py:230-250, one work chunk = up to CH particles of one (level, leaf block);
-// each particle's 3^d contributions are summed in a (4+2)^d LDS node tile, flushed with
-// float atomics once per chunk.
+constexpr int P2G_T = 256;       // threads per P2G workgroup
+constexpr int P2G_MAX_CH = 2048; // largest work chunk (particles of one leaf block)
+
+// adaptive_gimp.py:230-250.  One work chunk = up to CH particles of one (level, leaf block):
+//  1. the chunk is counting-sorted by cell inside the block (LDS histogram); the order is
+//     written back to perm, so G2P stores particles cell-sorted for the next substep;
+//  2. each lane computes its particle's 3^d node contributions; lanes of one cell are summed
+//     with a segmented DPP scan, and only the cell's last lane adds into the (4+2)^d LDS node
+//     tile (no two lanes of one instruction hit the same LDS word);
+//  3. the tile is flushed with one float atomic per non-zero node component.
 template <int D>
-__global__ void __launch_bounds__(256) k_p2g(Topo T, PBuf P, const int* perm, const int4* chunks,
-                                             const int* totals, Physics ph) {
+__global__ void __launch_bounds__(P2G_T) k_p2g(Topo T, PBuf P, int* perm, const int4* chunks, const int* totals,
+                                               Physics ph) {
   using PL = PLayout<D>;
   constexpr int TS = 6;
   constexpr int TN = (D == 2) ? TS * TS : TS * TS * TS;
+  constexpr int NC = Pow4<D>::v;  // cells per leaf block
+  constexpr int PER = P2G_MAX_CH / P2G_T;
   __shared__ float tile[TN * 4];
+  __shared__ int hist[NC];
+  __shared__ int sidx[P2G_MAX_CH];
   const int nchunks = totals[1];
+  const int tid = threadIdx.x;
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    int4 ck = chunks[c];
-    int l = level_of_gbid(T, ck.x);
+    const int4 ck = chunks[c];
+    const int l = level_of_gbid(T, ck.x);
     const LevelDesc& Ll = T.lv[l];
     int B[3];
     block_coords<D>(Ll.NB, ck.x - Ll.gbase, B);
-    for (int i = threadIdx.x; i < TN * 4; i += blockDim.x) tile[i] = 0.f;
+    for (int i = tid; i < TN * 4; i += P2G_T) tile[i] = 0.f;
+    if (tid < NC) hist[tid] = 0;
     __syncthreads();
-    const float scale = -ph.dt * Ll.inv_dx * ph.p_vol;  // -dt (0.5/r)^d / dx * V_p, r = 0.5
-    for (int i = threadIdx.x; i < ck.z; i += blockDim.x) {
-      int k = perm[ck.y + i];
-      float x[3], v[3], Cm[3][3], tau[6], mp;
+    // 1. counting sort of the chunk by cell
+    int myk[PER], mylc[PER], myrk[PER];
 #pragma unroll
-      for (int a = 0; a < D; ++a) {
-        x[a] = P.f[(size_t)(PL::X + a) * P.stride + k];
-        v[a] = P.f[(size_t)(PL::V + a) * P.stride + k];
+    for (int r = 0; r < PER; ++r) {
+      int i = r * P2G_T + tid;
+      bool valid = i < ck.z;
+      int k = 0, lc = 0;
+      if (valid) {
+        k = perm[ck.y + i];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          int b = base_cell(P.f[(size_t)(PL::X + a) * P.stride + k], Ll, T.pow2) + PAD - 4 * B[a];
+          lc = lc * 4 + b;
+        }
       }
+      myk[r] = k;
+      mylc[r] = lc;
+      myrk[r] = 0;
+      if (r * P2G_T < ck.z) myrk[r] = wave_run_atomic_add(&hist[lc], valid);
+    }
+    __syncthreads();
+    if (tid < 64) {  // exclusive scan of the NC counters by one wavefront
+      int v = (tid < NC) ? hist[tid] : 0, incl = v;
 #pragma unroll
-      for (int a = 0; a < D; ++a)
+      for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(incl, o);
+        if (tid >= o) incl += u;
+      }
+      if (tid < NC) hist[tid] = incl - v;
+    }
+    __syncthreads();
 #pragma unroll
-        for (int q = 0; q < D; ++q) Cm[a][q] = P.f[(size_t)(PL::C + a * D + q) * P.stride + k];
+    for (int r = 0; r < PER; ++r) {
+      int i = r * P2G_T + tid;
+      if (i < ck.z) {
+        int dst = hist[mylc[r]] + myrk[r];
+        sidx[dst] = myk[r];
+        perm[ck.y + dst] = myk[r];
+      }
+    }
+    __syncthreads();
+    // 2. contributions, reduced per cell
+    const float scale = -ph.dt * Ll.inv_dx * ph.p_vol;  // -dt (0.5/r)^d / dx * V_p with r = 0.5
+    for (int base = 0; base < ck.z; base += P2G_T) {
+      const int i = base + tid;
+      const bool valid = i < ck.z;
+      const int k = valid ? sidx[i] : 0;
+      float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f}, Cm[3][3] = {}, tau[6] = {}, mp = 0.f;
+      if (valid) {
 #pragma unroll
-      for (int q = 0; q < PL::NTAU; ++q) tau[q] = P.f[(size_t)(PL::TAU + q) * P.stride + k];
-      mp = P.f[(size_t)PL::M * P.stride + k];
-      int lb[3];
+        for (int a = 0; a < D; ++a) {
+          x[a] = P.f[(size_t)(PL::X + a) * P.stride + k];
+          v[a] = P.f[(size_t)(PL::V + a) * P.stride + k];
+        }
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+#pragma unroll
+          for (int q = 0; q < D; ++q) Cm[a][q] = P.f[(size_t)(PL::C + a * D + q) * P.stride + k];
+#pragma unroll
+        for (int q = 0; q < PL::NTAU; ++q) tau[q] = P.f[(size_t)(PL::TAU + q) * P.stride + k];
+        mp = P.f[(size_t)PL::M * P.stride + k];
+      }
+      int lb[3] = {0, 0, 0}, key = 0;
       float w[3][3], dw[3][3], tt[3];
 #pragma unroll
       for (int a = 0; a < D; ++a) {
         float qx = scaled(x[a], Ll, T.pow2);
         int b = (int)(qx + 0.5f) - 1;
         tt[a] = qx - (float)(b + 1);
-        lb[a] = b + PAD - 4 * B[a];
+        lb[a] = valid ? (b + PAD - 4 * B[a]) : 0;
+        key = key * 4 + lb[a];
         bspline(tt[a], w[a], dw[a]);
       }
+      if (!valid) key = 4096 + __lane_id();  // unique: never joins a neighbour's segment
+      const SegMask sm = seg_mask(key);
+      const bool writer = valid && sm.tail;
       float S[3][3];
 #pragma unroll
       for (int a = 0; a < D; ++a)
 #pragma unroll
         for (int q = 0; q < D; ++q) S[a][q] = scale * tau[tau_idx<D>(a, q)];
-      // v + C (i - (t+1)) = vb + C i
-      float vb[3];
+      float vb[3];  // v + C (i - (t+1)) = vb + C i
 #pragma unroll
       for (int a = 0; a < D; ++a) {
         float acc = v[a];
@@ -650,10 +754,15 @@ __global__ void __launch_bounds__(256) k_p2g(Topo T, PBuf P, const int* perm, co
             float mw = wt * mp;
             float p0 = mw * (vb[0] + Cm[0][0] * i0 + Cm[0][1] * i1) + S[0][0] * g0 + S[0][1] * g1;
             float p1 = mw * (vb[1] + Cm[1][0] * i0 + Cm[1][1] * i1) + S[1][0] * g0 + S[1][1] * g1;
-            int nd = ((lb[0] + i0) * TS + (lb[1] + i1)) * 4;
-            atomicAdd(&tile[nd + 0], p0);
-            atomicAdd(&tile[nd + 1], p1);
-            atomicAdd(&tile[nd + 3], mw);
+            p0 = seg_sum(p0, sm);
+            p1 = seg_sum(p1, sm);
+            mw = seg_sum(mw, sm);
+            if (writer) {
+              int nd = ((lb[0] + i0) * TS + (lb[1] + i1)) * 4;
+              atomicAdd(&tile[nd + 0], p0);
+              atomicAdd(&tile[nd + 1], p1);
+              atomicAdd(&tile[nd + 3], mw);
+            }
           }
       } else {
 #pragma unroll
@@ -673,16 +782,23 @@ __global__ void __launch_bounds__(256) k_p2g(Topo T, PBuf P, const int* perm, co
               for (int a = 0; a < 3; ++a)
                 p[a] = mw * (vb[a] + Cm[a][0] * i0 + Cm[a][1] * i1 + Cm[a][2] * i2) + S[a][0] * g0 + S[a][1] * g1 +
                        S[a][2] * g2;
-              int nd = (((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)) * 4;
-              atomicAdd(&tile[nd + 0], p[0]);
-              atomicAdd(&tile[nd + 1], p[1]);
-              atomicAdd(&tile[nd + 2], p[2]);
-              atomicAdd(&tile[nd + 3], mw);
+              p[0] = seg_sum(p[0], sm);
+              p[1] = seg_sum(p[1], sm);
+              p[2] = seg_sum(p[2], sm);
+              mw = seg_sum(mw, sm);
+              if (writer) {
+                int nd = (((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)) * 4;
+                atomicAdd(&tile[nd + 0], p[0]);
+                atomicAdd(&tile[nd + 1], p[1]);
+                atomicAdd(&tile[nd + 2], p[2]);
+                atomicAdd(&tile[nd + 3], mw);
+              }
             }
       }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < TN; i += blockDim.x) {
+    // 3. flush
+    for (int i = tid; i < TN; i += P2G_T) {
       float4 val = make_float4(tile[i * 4], tile[i * 4 + 1], tile[i * 4 + 2], tile[i * 4 + 3]);
       if (val.x == 0.f && val.y == 0.f && val.z == 0.f && val.w == 0.f) continue;
       int J[3];
@@ -1151,7 +1267,7 @@ void launch_bucket(hipStream_t s, const Topo& T, const PBuf& P, int n, SortBufs&
 }
 
 void launch_p2g(hipStream_t s, const Topo& T, const PBuf& Pin, SortBufs& S, const Physics& ph, int p2g_grid) {
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_p2g<DD>, dim3(p2g_grid), dim3(256), 0, s, T, Pin, S.perm, S.chunks, S.totals,
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_p2g<DD>, dim3(p2g_grid), dim3(P2G_T), 0, s, T, Pin, S.perm, S.chunks, S.totals,
                                       ph));
 }
 

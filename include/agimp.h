@@ -54,7 +54,7 @@ typedef struct agimp_config {
   int32_t slab_enable;     /* sticky slab collider on grid nodes */
   int32_t slab_axis;
   float slab_lo, slab_hi;
-  int32_t particles_per_chunk; /* P2G work chunk (particles of one leaf block), 64..4096 */
+  int32_t particles_per_chunk; /* P2G work chunk (particles of one leaf block), 16..2048 */
   int32_t device;          /* HIP device ordinal */
 } agimp_config;
 

@@ -7,7 +7,8 @@ Bars:
     own raw bits there depend on a race, adaptive_gimp.py:174 vs :182-183);
   * FP32 state: grid values after one substep within REL_GRID of the level's largest
     magnitude; particle v and C within REL_P of the field's largest magnitude, x within
-    ABS_X, F within ABS_F.  Summation order differs (float atomics, gathers), so FP32
+    ABS_X, F within ABS_F; C (a velocity gradient in cell units) within 10 REL_P of the
+    largest velocity.  Summation order differs (float atomics, gathers), so FP32
     rounding, not algorithm, sets these bounds.
 """
 import numpy as np
@@ -73,8 +74,10 @@ def particle_errors(sim, ora):
     r = ora.get_particles()
     g = {k: getattr(sim, k).to_numpy() for k in ("x_p", "v_p", "F_p", "C_p", "fl_p", "Jp_p")}
     rel = lambda a, b: np.abs(a - b).max() / max(np.abs(b).max(), 1e-30)
+    # C = sum_i v_i (x) grad w_i (cell units): its rounding error scales with |v|, not |C|
+    vscale = max(np.abs(r["v"]).max(), np.abs(r["C"]).max(), 1e-30)
     return dict(fl=int((g["fl_p"] != r["fl"]).sum()), x=float(np.abs(g["x_p"] - r["x"]).max()),
-                v=float(rel(g["v_p"], r["v"])), C=float(rel(g["C_p"], r["C"])),
+                v=float(rel(g["v_p"], r["v"])), C=float(np.abs(g["C_p"] - r["C"]).max() / vscale),
                 F=float(np.abs(g["F_p"] - r["F"]).max()), Jp=float(np.abs(g["Jp_p"] - r["Jp"]).max()))
 
 
