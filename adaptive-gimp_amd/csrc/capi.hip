@@ -154,21 +154,16 @@ size_t flag_words(const Topo& T) {
   return (flags + 3) / 4;
 }
 
-// zero the grid blocks the previous physics step wrote, and empty the block lists
-void clear_prev(agimp_handle* h) {
-  if (h->have_slists) launch_clear_sgrid(h->stream, h->T, h->grid_grid);
-  launch_fill_u32(h->stream, h->T.scount, 0, MAXL);
-  h->have_slists = 0;
-}
-
 int do_refine(agimp_handle* h) {
   Topo& T = h->T;
   PhaseScope ps(h, AGIMP_PHASE_REFINE);
-  clear_prev(h);
+  if (h->have_slists) launch_clear_sgrid(h->stream, T, h->grid_grid);
+  h->have_slists = 0;
   // cell masks of all levels are one allocation, as are the block flags
-  launch_fill_u32(h->stream, T.lv[0].cell, 0, cells_total(T, 0, T.L - 1) / 4);
-  launch_fill_u32(h->stream, T.node, 0, (size_t)T.lv[T.L - 1].nblocks * (h->D == 2 ? 16 : 64) / 4);
-  launch_fill_u32(h->stream, T.lv[0].sflag, 0, flag_words(T));
+  void* ptrs[6] = {T.lv[0].cell, T.node, T.lv[0].sflag, T.cflag, T.scount, T.ccount};
+  size_t words[6] = {cells_total(T, 0, T.L - 1) / 4, (size_t)T.lv[T.L - 1].nblocks * (h->D == 2 ? 16 : 64) / 4,
+                     flag_words(T), flag_words(T), (size_t)MAXL, 4};
+  launch_fill_ranges(h->stream, ptrs, words, 6);
   launch_refine(h->stream, T, h->pa, h->n, h->static_mode, h->st_ret, h->st_act, h->counters);
   // blocks holding valid nodes of the new octree, kept for fixed-topology substeps
   launch_copy_u32(h->stream, h->vflag, T.lv[0].sflag, flag_words(T));
@@ -181,10 +176,12 @@ int do_physics(agimp_handle* h, float dt) {
   Topo& T = h->T;
   {
     PhaseScope ps(h, AGIMP_PHASE_BUCKET);
-    clear_prev(h);
+    if (h->have_slists) launch_clear_sgrid(h->stream, T, h->grid_grid);
+    h->have_slists = 0;
+    void* ptrs[3] = {h->S.bcnt, h->S.frozen, T.scount};
+    size_t words[3] = {(size_t)T.total_blocks, 1, (size_t)MAXL};
+    launch_fill_ranges(h->stream, ptrs, words, 3);
     launch_copy_u32(h->stream, T.lv[0].sflag, h->vflag, flag_words(T));
-    launch_fill_u32(h->stream, h->S.bcnt, 0, (size_t)T.total_blocks);
-    launch_fill_u32(h->stream, h->S.frozen, 0, 1);
     launch_bucket(h->stream, T, h->pa, h->n, h->S, h->CH, h->counters);
   }
   Physics ph = h->ph;
@@ -297,6 +294,9 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   rc = rc ? rc : dalloc(h, &h->vflag, (flags + 3) / 4);
   rc = rc ? rc : dalloc(h, &T.node, (size_t)T.lv[T.L - 1].nblocks * p4);
   rc = rc ? rc : dalloc(h, &T.scount, MAXL);
+  rc = rc ? rc : dalloc(h, &T.cflag, (flags + 3) & ~(size_t)3);
+  rc = rc ? rc : dalloc(h, &T.clist, flags);
+  rc = rc ? rc : dalloc(h, &T.ccount, 4);
   if (rc) { agimp_destroy(h); return rc; }
   size_t co = 0, fo = 0;
   for (int l = 0; l < T.L; ++l) {

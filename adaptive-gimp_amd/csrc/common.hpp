@@ -43,6 +43,9 @@ struct Topo {
   LevelDesc lv[MAXL];
   uint8_t* node;     // node_mask, finest layout
   int* scount;       // [MAXL] number of entries in each slist
+  uint8_t* cflag;    // [total_blocks] leaf block holds a non-zero cell (by global block id)
+  int* clist;        // [total_blocks] compacted cflag (global block ids)
+  int* ccount;       // [4] list lengths of the successive compactions of cflag
 };
 
 struct Physics {

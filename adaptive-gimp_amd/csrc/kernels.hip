@@ -43,33 +43,33 @@ __global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, size_t n) {
   for (; i < n; i += st) dst[i] = src[i];
 }
 
-// decode a flat cell/node index over the concatenated per-level arrays
+// cell t of a compacted block list (global block ids): level, coordinates and index
 template <int D>
-__device__ __forceinline__ bool decode_flat(const Topo& T, int lmin, int lmax, size_t t, int& l, int* I, uint32_t& local) {
-  size_t acc = 0;
-  for (int q = 0; q < T.L; ++q) {
-    if (q < lmin || q > lmax) continue;
-    size_t cnt = (size_t)T.lv[q].nblocks * Pow4<D>::v;
-    if (t < acc + cnt) {
-      l = q;
-      local = (uint32_t)(t - acc);
-      int b = (int)(local / Pow4<D>::v);
-      int w = (int)(local % Pow4<D>::v);
-      int B[3];
-      block_coords<D>(T.lv[q].NB, b, B);
-      if (D == 2) {
-        I[0] = B[0] * 4 + (w >> 2) - PAD;
-        I[1] = B[1] * 4 + (w & 3) - PAD;
-      } else {
-        I[0] = B[0] * 4 + (w >> 4) - PAD;
-        I[1] = B[1] * 4 + ((w >> 2) & 3) - PAD;
-        I[2] = B[2] * 4 + (w & 3) - PAD;
-      }
-      return true;
-    }
-    acc += cnt;
+__device__ __forceinline__ bool decode_list(const Topo& T, const int* list, int count, int t, int& l, int* I,
+                                            uint32_t& q) {
+  int e = t / Pow4<D>::v, w = t % Pow4<D>::v;
+  if (e >= count) return false;
+  int g = list[e];
+  l = level_of_gbid(T, g);
+  int b = g - T.lv[l].gbase;
+  int B[3];
+  block_coords<D>(T.lv[l].NB, b, B);
+  if (D == 2) {
+    I[0] = B[0] * 4 + (w >> 2) - PAD;
+    I[1] = B[1] * 4 + (w & 3) - PAD;
+  } else {
+    I[0] = B[0] * 4 + (w >> 4) - PAD;
+    I[1] = B[1] * 4 + ((w >> 2) & 3) - PAD;
+    I[2] = B[2] * 4 + (w & 3) - PAD;
   }
-  return false;
+  q = (uint32_t)b * Pow4<D>::v + w;
+  return true;
+}
+
+// flag the leaf block holding cell q of level l as non-empty
+template <int D>
+__device__ __forceinline__ void mark_cell_block(const Topo& T, int l, int64_t q) {
+  T.cflag[T.lv[l].gbase + (int)(q / Pow4<D>::v)] = 1;
 }
 
 // index of level-l coordinates I (unpadded), or -1 if outside the padded box
@@ -115,6 +115,33 @@ __device__ __forceinline__ void mark_valid_node(const Topo& T, const int* f) {
   }
 }
 
+// zero several buffers in one launch
+struct FillRanges {
+  uint32_t* p[8];
+  size_t n[8];
+  int k;
+};
+
+__global__ void k_fill_ranges(FillRanges R) {
+  size_t total = 0;
+  for (int r = 0; r < R.k; ++r) total += R.n[r];
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += st) {
+    size_t j = i;
+    int r = 0;
+    while (j >= R.n[r]) { j -= R.n[r]; ++r; }
+    R.p[r][j] = 0u;
+  }
+}
+
+// append the ids of set flags to list (order unspecified)
+__global__ void k_compact_flags(const uint8_t* flags, int n, int* list, int* count) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  bool on = g < n && flags[g] != 0;
+  int idx = wave_run_atomic_add(count, on);
+  if (on) list[idx] = g;
+}
+
 // ------------------------------------------------------------------------------ clear
 template <int D>
 __global__ void k_clear_sgrid(Topo T) {
@@ -156,6 +183,7 @@ __global__ void k_activate_dynamic(Topo T, PBuf P, int n, unsigned long long* cn
     int64_t q = lidx<D>(Lg, I);
     if (q < 0) { count(cnt, CNT_OOB); continue; }
     Lg.cell[q] = ACTIVATED;  // cells only ever receive ACTIVATED in this phase
+    mark_cell_block<D>(T, g, q);
   }
 }
 
@@ -177,8 +205,12 @@ __device__ __forceinline__ int static_gi(const Topo& T, const int8_t* ret, const
 #pragma unroll
     for (int a = 0; a < D; ++a) P[a] = I[a] >> (T.L - 1 - al);
     int64_t q = lidx<D>(T.lv[al], P);
-    if (q < 0) count(cnt, CNT_OOB);
-    else T.lv[al].cell[q] = ACTIVATED;
+    if (q < 0) {
+      count(cnt, CNT_OOB);
+    } else {
+      T.lv[al].cell[q] = ACTIVATED;
+      mark_cell_block<D>(T, al, q);
+    }
   }
   return ret[r];
 }
@@ -220,63 +252,69 @@ __global__ void k_activate_static(Topo T, PBuf P, int n, const int8_t* ret, cons
 // ------------------------------------------------------------------------------ culling
 // adaptive_gimp.py:98-109 for all l >= 1 at once (ACTIVATED bits do not change here)
 template <int D>
-__global__ void k_mark_hier(Topo T, size_t total, unsigned long long* cnt) {
-  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  int l, I[3];
-  uint32_t q;
-  if (!decode_flat<D>(T, 1, T.L - 1, t, l, I, q)) return;
-  if (!(T.lv[l].cell[q] & ACTIVATED)) return;
-  int l0 = -1;
-  for (int lq = 0; lq < l; ++lq) {
-    int P[3];
+__global__ void k_mark_hier(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+  const int nlist = *countp;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
+    int l, I[3];
+    uint32_t q;
+    if (!decode_list<D>(T, list, nlist, t, l, I, q) || l < 1) continue;
+    if (!(T.lv[l].cell[q] & ACTIVATED)) continue;
+    int l0 = -1;
+    for (int lq = 0; lq < l; ++lq) {
+      int P[3];
 #pragma unroll
-    for (int a = 0; a < D; ++a) P[a] = I[a] >> (l - lq);
-    int64_t qq = lidx<D>(T.lv[lq], P);
-    if (qq < 0) { count(cnt, CNT_OOB); continue; }
-    if (T.lv[lq].cell[qq] & ACTIVATED) l0 = lq;
-  }
-  if (l0 < 0) return;
-  for (int lq = l0; lq < l; ++lq) {
-    int P[3];
+      for (int a = 0; a < D; ++a) P[a] = I[a] >> (l - lq);
+      int64_t qq = lidx<D>(T.lv[lq], P);
+      if (qq < 0) { count(cnt, CNT_OOB); continue; }
+      if (T.lv[lq].cell[qq] & ACTIVATED) l0 = lq;
+    }
+    if (l0 < 0) continue;
+    for (int lq = l0; lq < l; ++lq) {
+      int P[3];
 #pragma unroll
-    for (int a = 0; a < D; ++a) P[a] = I[a] >> (l - lq);
-    int64_t qq = lidx<D>(T.lv[lq], P);
-    if (qq >= 0) T.lv[lq].cell[qq] |= CULLING;
+      for (int a = 0; a < D; ++a) P[a] = I[a] >> (l - lq);
+      int64_t qq = lidx<D>(T.lv[lq], P);
+      if (qq >= 0) {
+        T.lv[lq].cell[qq] |= CULLING;
+        mark_cell_block<D>(T, lq, qq);
+      }
+    }
   }
 }
 
 // adaptive_gimp.py:111-118 for all l at once: children of CULLING cells get ACTIVATED.
 // The culled cell itself keeps A|C (it compares unequal to ACTIVATED from here on).
 template <int D>
-__global__ void k_split(Topo T, size_t total, unsigned long long* cnt) {
-  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  int l, I[3];
-  uint32_t q;
-  if (!decode_flat<D>(T, 0, T.L - 2, t, l, I, q)) return;
-  if (!(T.lv[l].cell[q] & CULLING)) return;
+__global__ void k_split(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+  const int nlist = *countp;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
+    int l, I[3];
+    uint32_t q;
+    if (!decode_list<D>(T, list, nlist, t, l, I, q) || l > T.L - 2) continue;
+    if (!(T.lv[l].cell[q] & CULLING)) continue;
 #pragma unroll
-  for (int u = 0; u < (1 << D); ++u) {
-    int Jc[3];
+    for (int u = 0; u < (1 << D); ++u) {
+      int Jc[3];
 #pragma unroll
-    for (int a = 0; a < D; ++a) Jc[a] = 2 * I[a] + ((u >> (D - 1 - a)) & 1);
-    int64_t qq = lidx<D>(T.lv[l + 1], Jc);
-    if (qq < 0) { count(cnt, CNT_OOB); continue; }
-    T.lv[l + 1].cell[qq] |= ACTIVATED;
+      for (int a = 0; a < D; ++a) Jc[a] = 2 * I[a] + ((u >> (D - 1 - a)) & 1);
+      int64_t qq = lidx<D>(T.lv[l + 1], Jc);
+      if (qq < 0) { count(cnt, CNT_OOB); continue; }
+      T.lv[l + 1].cell[qq] |= ACTIVATED;
+      mark_cell_block<D>(T, l + 1, qq);
+    }
   }
 }
 
 // ------------------------------------------------------------------------------ node marks
 // adaptive_gimp.py:142-153: corners of exactly-ACTIVATED cells get ACTIVATED
 template <int D>
-__global__ void k_node_A(Topo T, size_t total, unsigned long long* cnt) {
-  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= total) return;
+__global__ void k_node_A(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+  const int nlist = *countp;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
   uint32_t q;
-  if (!decode_flat<D>(T, 0, T.L - 1, t, l, I, q)) return;
-  if (T.lv[l].cell[q] != ACTIVATED) return;
+  if (!decode_list<D>(T, list, nlist, t, l, I, q)) continue;
+  if (T.lv[l].cell[q] != ACTIVATED) continue;
 #pragma unroll
   for (int u = 0; u < (1 << D); ++u) {
     int Jn[3], f[3];
@@ -286,6 +324,7 @@ __global__ void k_node_A(Topo T, size_t total, unsigned long long* cnt) {
     if (qq < 0) { count(cnt, CNT_OOB); continue; }
     T.node[qq] = ACTIVATED;  // node_mask only receives ACTIVATED in this phase
     mark_valid_node<D>(T, f);
+  }
   }
 }
 
@@ -315,13 +354,13 @@ __device__ __forceinline__ int face_parent(const Topo& T, int l, const int* I, i
 
 // adaptive_gimp.py:170-174 for all levels at once
 template <int D>
-__global__ void k_mark_T(Topo T, size_t total, unsigned long long* cnt) {
-  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= total) return;
+__global__ void k_mark_T(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+  const int nlist = *countp;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
   uint32_t q;
-  if (!decode_flat<D>(T, 0, T.L - 1, t, l, I, q)) return;
-  if (T.lv[l].cell[q] != ACTIVATED) return;
+  if (!decode_list<D>(T, list, nlist, t, l, I, q)) continue;
+  if (T.lv[l].cell[q] != ACTIVATED) continue;
   for (int j = 0; j < 2 * D; ++j) {
     int I1[3];
     int lp = face_parent<D>(T, l, I, j, I1);
@@ -349,17 +388,18 @@ __global__ void k_mark_T(Topo T, size_t total, unsigned long long* cnt) {
       mark_valid_node<D>(T, f);
     }
   }
+  }
 }
 
 // adaptive_gimp.py:176-183 for all levels at once, after every T mark
 template <int D>
-__global__ void k_mark_G(Topo T, size_t total, unsigned long long* cnt) {
-  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= total) return;
+__global__ void k_mark_G(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+  const int nlist = *countp;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
   uint32_t q;
-  if (!decode_flat<D>(T, 0, T.L - 1, t, l, I, q)) return;
-  if (T.lv[l].cell[q] != ACTIVATED) return;
+  if (!decode_list<D>(T, list, nlist, t, l, I, q)) continue;
+  if (T.lv[l].cell[q] != ACTIVATED) continue;
   for (int j = 0; j < 2 * D; ++j) {
     int I1[3];
     int lp = face_parent<D>(T, l, I, j, I1);
@@ -384,6 +424,7 @@ __global__ void k_mark_G(Topo T, size_t total, unsigned long long* cnt) {
         }
       }
     }
+  }
   }
 }
 
@@ -1227,25 +1268,47 @@ void launch_clear_sgrid(hipStream_t s, const Topo& T, int grid_blocks) {
   AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_clear_sgrid<DD>, dim3(grid_blocks), dim3(256), 0, s, T));
 }
 
+void launch_fill_ranges(hipStream_t s, void* const* ptrs, const size_t* words, int k) {
+  FillRanges R{};
+  size_t total = 0;
+  R.k = 0;
+  for (int i = 0; i < k && i < 8; ++i) {
+    if (words[i] == 0) continue;
+    R.p[R.k] = (uint32_t*)ptrs[i];
+    R.n[R.k] = words[i];
+    total += words[i];
+    ++R.k;
+  }
+  if (total == 0) return;
+  unsigned g = (unsigned)std::min<size_t>(blocks_for(total, 256), 8192);
+  hipLaunchKernelGGL(k_fill_ranges, dim3(g), dim3(256), 0, s, R);
+}
+
+static void compact_cells(hipStream_t s, const Topo& T, int which) {
+  hipLaunchKernelGGL(k_compact_flags, dim3(blocks_for(T.total_blocks, 256)), dim3(256), 0, s, T.cflag,
+                     T.total_blocks, T.clist, T.ccount + which);
+}
+
 void launch_refine(hipStream_t s, const Topo& T, const PBuf& P, int n, int static_mode, const int8_t* ret,
                    const int8_t* act, unsigned long long* cnt) {
   unsigned pb = blocks_for(n, 256);
+  const unsigned lg = 2048;  // grid-stride launches over block lists of device-side length
   AGIMP_DIM(T.dim, {
     if (static_mode) {
       hipLaunchKernelGGL(k_activate_static<DD>, dim3(pb), dim3(256), 0, s, T, P, n, ret, act, cnt);
     } else {
       hipLaunchKernelGGL(k_activate_dynamic<DD>, dim3(pb), dim3(256), 0, s, T, P, n, cnt);
       if (T.L > 1) {
-        size_t tm = cells_total(T, 1, T.L - 1);
-        hipLaunchKernelGGL(k_mark_hier<DD>, dim3(blocks_for(tm, 256)), dim3(256), 0, s, T, tm, cnt);
-        size_t ts = cells_total(T, 0, T.L - 2);
-        hipLaunchKernelGGL(k_split<DD>, dim3(blocks_for(ts, 256)), dim3(256), 0, s, T, ts, cnt);
+        compact_cells(s, T, 0);
+        hipLaunchKernelGGL(k_mark_hier<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 0, cnt);
+        compact_cells(s, T, 1);
+        hipLaunchKernelGGL(k_split<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 1, cnt);
       }
     }
-    size_t ta = cells_total(T, 0, T.L - 1);
-    hipLaunchKernelGGL(k_node_A<DD>, dim3(blocks_for(ta, 256)), dim3(256), 0, s, T, ta, cnt);
-    hipLaunchKernelGGL(k_mark_T<DD>, dim3(blocks_for(ta, 256)), dim3(256), 0, s, T, ta, cnt);
-    hipLaunchKernelGGL(k_mark_G<DD>, dim3(blocks_for(ta, 256)), dim3(256), 0, s, T, ta, cnt);
+    compact_cells(s, T, 2);
+    hipLaunchKernelGGL(k_node_A<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 2, cnt);
+    hipLaunchKernelGGL(k_mark_T<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 2, cnt);
+    hipLaunchKernelGGL(k_mark_G<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 2, cnt);
   });
 }
 

@@ -23,6 +23,8 @@ struct SortBufs {
 
 size_t cells_total(const Topo& T, int lmin, int lmax);
 void launch_fill_u32(hipStream_t s, void* p, uint32_t v, size_t n);
+// zero up to 8 buffers (ptrs[i], words[i] 32-bit words) in one launch
+void launch_fill_ranges(hipStream_t s, void* const* ptrs, const size_t* words, int k);
 void launch_copy_u32(hipStream_t s, void* dst, const void* src, size_t n);
 void launch_clear_sgrid(hipStream_t s, const Topo& T, int grid_blocks);
 void launch_refine(hipStream_t s, const Topo& T, const PBuf& P, int n, int static_mode, const int8_t* ret,

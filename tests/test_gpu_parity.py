@@ -168,7 +168,10 @@ def test_one_substep_grid_and_particles(pkg, O, name):
     sim.close()
 
 
-@pytest.mark.parametrize("name", SCENES)
+# c1 as specified (rho=1, E=1e4, dt=1e-4) violates the P-wave CFL bound: round-off grows
+# exponentially (test_oracle_pins.test_c1_as_specified_is_unstable), so it is compared over one
+# substep only.
+@pytest.mark.parametrize("name", [s for s in SCENES if s != "c1_two_squares"])
 def test_many_substeps_track_oracle(pkg, O, name):
     """10 substeps (stress and affine terms active from step 2), then the topology of both
     rebuilt from the GPU's particle snapshot must agree bit for bit."""

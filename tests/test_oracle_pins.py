@@ -364,3 +364,24 @@ def test_uniform_velocity_free_fall_multilevel(O, pkg):
     assert ok.all()
     np.testing.assert_allclose(got["v"][ok], np.tile(v0 + np.array([0.0, -10.0]) * dt, (ok.sum(), 1)), atol=1e-12)
     np.testing.assert_allclose(got["C"][ok], 0, atol=1e-11)
+
+
+def test_c1_as_specified_is_unstable(O, pkg):
+    """The 2D two-squares workload with rho=1, E=1e4, dt=1e-4 has a P-wave speed of ~105, so
+    dt exceeds dx/c = 3.7e-5: the oracle's own FP32 and FP64 runs agree to 1e-9 after one
+    substep and then separate exponentially.  Multi-substep parity is meaningless there."""
+    wl = pkg.scenes.c1_two_squares()
+    st = wl.state
+
+    def run(fp64, steps):
+        o = O.OracleSim(2, wl.level, wl.coarsest_size, wl.n, fp64=fp64, **wl.params)
+        o.set_particles(st["x"].astype(np.float32), st["v"], st["F"], st["C"], st["m"].astype(np.float32), st["g"])
+        for _ in range(steps):
+            o.substep(wl.dt)
+        return o.get_particles()["v"]
+
+    one = np.abs(run(False, 1) - run(True, 1)).max()
+    ten32, ten64 = run(False, 10), run(True, 10)
+    assert one < 1e-8
+    assert np.abs(ten32 - ten64).max() > 0.1 * np.abs(ten64).max()
+    assert np.abs(ten64).max() > 100 * np.abs(st["v"]).max() + 1.0  # blow-up from rest
