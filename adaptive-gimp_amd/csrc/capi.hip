@@ -231,7 +231,7 @@ int agimp_config_init(agimp_config* c, int32_t dim, int32_t level, int32_t coars
   c->theta_s = 7.5e-3f;
   c->hardening = 0.0f;
   c->slab_axis = 1;
-  c->particles_per_chunk = 256;
+  c->particles_per_chunk = 1024;
   c->device = 0;
   return 0;
 }

@@ -103,7 +103,7 @@ class AdaptiveGIMP:
     def __init__(self, dim, level, coarsest_size, n_particles, particle_initializer, grid_initializer=None, *,
                  E=5e3, nu=0.2, gravity=None, p_mass=1.0, p_vol=1.0, boundary_gap=4, wall_mode="slip",
                  plastic=False, theta_c=2.5e-2, theta_s=7.5e-3, hardening=0.0, slab=None,
-                 particles_per_chunk=256, device=0):
+                 particles_per_chunk=1024, device=0):
         if dim not in (2, 3):
             raise ValueError("dim must be 2 or 3")
         self.dim = dim

@@ -26,6 +26,18 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a known-size copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["kernels"][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def algorithmic_bytes(dim, n_particles, n_nodes):
@@ -72,7 +84,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ppc", type=int, default=256, help="P2G particles per chunk")
+    ap.add_argument("--ppc", type=int, default=1024, help="P2G particles per chunk")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run one plain device copy of this many MiB (a known-size kernel that calibrates "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU layout is a "
@@ -84,6 +99,16 @@ def main():
     import __graft_entry__ as ge
 
     pkg = ge.package()
+    if args.copy_mb > 0:
+        import torch
+
+        a = torch.empty(args.copy_mb * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     wl = pkg.scenes.c3_sphere_on_slab(n=args.n)
     sim = pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(),
                            particles_per_chunk=args.ppc, **wl.params)
@@ -129,7 +154,7 @@ def main():
         "config": {"workload": wl.name, "dim": wl.dim, "levels": wl.level, "coarsest_size": wl.coarsest_size,
                    "n_particles": wl.n, "dt": wl.dt, "mode": "plain", "particles_per_chunk": args.ppc},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                      "bytes_per_launch": kern[dom][0], "avg_kernel_ms": kern[dom][1], "active_nodes": n_nodes},
         "phase_ms": phase_ms,
         "p2g_gbs": b_p2g / (avg["p2g"] * 1e-3) / 1e9,
