@@ -5,6 +5,6 @@ package is the Python host side mirroring ``adaptive_gimp.py``'s ``AdaptiveGIMP`
 """
 from . import scenes
 from ._lib import AgimpError
-from .sim import ACTIVATED, CULLING, GHOST, T_JUNCTION, UNACTIVATED, AdaptiveGIMP
+from .sim import ACTIVATED, CULLING, GHOST, T_JUNCTION, UNACTIVATED, AdaptiveGIMP, Batch
 
-__all__ = ["AdaptiveGIMP", "AgimpError", "scenes", "UNACTIVATED", "ACTIVATED", "GHOST", "T_JUNCTION", "CULLING"]
+__all__ = ["AdaptiveGIMP", "Batch", "AgimpError", "scenes", "UNACTIVATED", "ACTIVATED", "GHOST", "T_JUNCTION", "CULLING"]

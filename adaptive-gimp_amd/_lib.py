@@ -37,8 +37,12 @@ EXPORTS = [
     "agimp_config_init", "agimp_create", "agimp_destroy", "agimp_set_particles", "agimp_set_static_levels",
     "agimp_refine", "agimp_substep", "agimp_substep_fixed_topology", "agimp_advance", "agimp_get_particles",
     "agimp_get_level", "agimp_get_node_mask", "agimp_get_counters", "agimp_reset_counters",
-    "agimp_level_extent", "agimp_set_timing", "agimp_get_timings", "agimp_sync", "agimp_last_error", "agimp_host_svd", "agimp_host_constitutive",
+    "agimp_level_extent", "agimp_set_timing", "agimp_get_timings", "agimp_sync", "agimp_last_error",
+    "agimp_set_particles_per_chunk", "agimp_batch_create", "agimp_batch_destroy",
+    "agimp_batch_set_particles_per_chunk", "agimp_batch_advance", "agimp_batch_last_error",
+    "agimp_host_svd", "agimp_host_constitutive",
 ]
+MIN_CHUNK, MAX_CHUNK = 64, 2048
 
 _lib = None
 
@@ -82,13 +86,19 @@ def load():
         "agimp_get_timings": [vp, dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32],
         "agimp_sync": [vp],
         "agimp_last_error": [vp],
+        "agimp_set_particles_per_chunk": [vp, ctypes.c_int32],
+        "agimp_batch_create": [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)],
+        "agimp_batch_destroy": [vp],
+        "agimp_batch_set_particles_per_chunk": [vp, ip],
+        "agimp_batch_advance": [vp, fp, ctypes.c_int32],
+        "agimp_batch_last_error": [vp],
         "agimp_host_svd": [ctypes.c_int32, dp, dp, dp, dp],
         "agimp_host_constitutive": [cfgp, dp, dp, dp, ctypes.c_int32],
     }
     for name, args in sig.items():
         f = getattr(L, name)
         f.argtypes = args
-        f.restype = ctypes.c_char_p if name == "agimp_last_error" else ctypes.c_int
+        f.restype = ctypes.c_char_p if name.endswith("last_error") else ctypes.c_int
     _lib = L
     return L
 

@@ -1,5 +1,5 @@
-// C-ABI entry points (include/agimp.h): device state of one simulator and the host-side
-// orchestration of a substep (adaptive_gimp.py:323-352).
+// C-ABI entry points (include/agimp.h): device state of one simulator, batches of
+// simulators, and the host-side orchestration of a substep (adaptive_gimp.py:323-352).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,26 +16,34 @@
 
 using namespace agimp;
 
+// One captured substep (refine + bucketing + physics) of a set of scenes, replayed with
+// hipGraphLaunch.  Kernels read dt and the particle-buffer parity from device memory, so
+// the graph stays valid when dt changes.
+struct Captured {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
 struct agimp_handle {
   agimp_config cfg{};
   int D = 0, L = 0;
   int n = 0;
   hipStream_t stream = nullptr;
-  Topo T{};
-  Physics ph{};
-  PBuf pa{}, pb{};  // current / next particle buffers
-  SortBufs S{};
-  unsigned long long* counters = nullptr;
-  uint32_t* vflag = nullptr;  // block flags of valid nodes of the current octree
-  int8_t* st_ret = nullptr;
-  int8_t* st_act = nullptr;
-  int static_mode = 0;
+  SceneDesc hd{};             // host copy of the scene descriptor
+  SceneDesc* dd = nullptr;    // device copy (one entry)
+  Dims dims{};
+  int host_parity = 0;        // mirrors *hd.parity
   int has_particles = 0;
-  int have_slists = 0;     // grid blocks of the last substep are listed (to be cleared)
-  int topo_valid = 0;      // masks describe the current octree
-  int CH = 256;
+  int topo_valid = 0;
   int max_chunks = 0;
-  int p2g_grid = 0, grid_grid = 0;
+  float graph_dt = -1.f;      // dt stored in the device descriptor for graph replays
+  Captured graph;
   std::vector<void*> allocs;
   std::string err;
   // phase timing with HIP events on the handle's stream (agimp_set_timing)
@@ -45,6 +53,16 @@ struct agimp_handle {
   std::vector<hipEvent_t> event_pool;
   double phase_ms[AGIMP_NPHASES] = {};
   int64_t phase_calls[AGIMP_NPHASES] = {};
+};
+
+struct agimp_batch {
+  std::vector<agimp_handle*> scenes;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  SceneDesc* dd = nullptr;
+  Dims dims{};
+  Captured graph;
+  std::string err;
 };
 
 namespace {
@@ -85,19 +103,26 @@ Physics make_physics(const agimp_config& c) {
   p.theta_s = c.theta_s;
   p.hardening = c.hardening;
   double E = c.E, nu = c.nu;
-  p.mu0 = E / (2.0 * (1.0 + nu));                    // adaptive_gimp.py:53
+  p.mu0 = E / (2.0 * (1.0 + nu));  // adaptive_gimp.py:53
   p.la0 = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   p.p_vol = c.p_vol;
   p.slab_enable = c.slab_enable;
   p.slab_axis = c.slab_axis;
   p.slab_lo = c.slab_lo;
   p.slab_hi = c.slab_hi;
+  p.dt = 0.f;
   return p;
 }
 
 int check_ready(agimp_handle* h) {
   if (!h) return AGIMP_EINVAL;
   if (!h->has_particles) return fail(h, AGIMP_ESTATE, "particles not set (agimp_set_particles)");
+  return 0;
+}
+
+int upload_desc(agimp_handle* h) {
+  HIPCHK(h, hipMemcpyAsync(h->dd, &h->hd, sizeof(SceneDesc), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
 }
 
@@ -148,64 +173,71 @@ int collect_marks(agimp_handle* h) {
   return 0;
 }
 
-size_t flag_words(const Topo& T) {
-  size_t flags = 0;
-  for (int l = 0; l < T.L; ++l) flags += (size_t)T.lv[l].nblocks;
-  return (flags + 3) / 4;
+Dims single_dims(const agimp_handle* h) {
+  Dims m{};
+  m.dim = h->D;
+  m.nscenes = 1;
+  m.maxL = h->L;
+  m.any_static = h->hd.static_mode;
+  m.any_dynamic = !h->hd.static_mode;
+  m.max_n = (size_t)h->n;
+  m.max_total_blocks = (size_t)h->hd.T.total_blocks;
+  m.max_fill_words = h->hd.w_cells + h->hd.w_node + 2 * h->hd.w_flags;
+  m.list_grid = 2048;
+  m.p2g_grid = std::max(1, std::min(h->max_chunks, 8192));
+  m.grid_grid = 4096;
+  return m;
 }
 
 int do_refine(agimp_handle* h) {
-  Topo& T = h->T;
   PhaseScope ps(h, AGIMP_PHASE_REFINE);
-  if (h->have_slists) launch_clear_sgrid(h->stream, T, h->grid_grid);
-  h->have_slists = 0;
-  // cell masks of all levels are one allocation, as are the block flags
-  void* ptrs[6] = {T.lv[0].cell, T.node, T.lv[0].sflag, T.cflag, T.scount, T.ccount};
-  size_t words[6] = {cells_total(T, 0, T.L - 1) / 4, (size_t)T.lv[T.L - 1].nblocks * (h->D == 2 ? 16 : 64) / 4,
-                     flag_words(T), flag_words(T), (size_t)MAXL, 4};
-  launch_fill_ranges(h->stream, ptrs, words, 6);
-  launch_refine(h->stream, T, h->pa, h->n, h->static_mode, h->st_ret, h->st_act, h->counters);
-  // blocks holding valid nodes of the new octree, kept for fixed-topology substeps
-  launch_copy_u32(h->stream, h->vflag, T.lv[0].sflag, flag_words(T));
+  launch_refine(h->stream, h->dd, h->dims);
   h->topo_valid = 1;
   HIPCHK(h, hipGetLastError());
   return 0;
 }
 
 int do_physics(agimp_handle* h, float dt) {
-  Topo& T = h->T;
   {
     PhaseScope ps(h, AGIMP_PHASE_BUCKET);
-    if (h->have_slists) launch_clear_sgrid(h->stream, T, h->grid_grid);
-    h->have_slists = 0;
-    void* ptrs[3] = {h->S.bcnt, h->S.frozen, T.scount};
-    size_t words[3] = {(size_t)T.total_blocks, 1, (size_t)MAXL};
-    launch_fill_ranges(h->stream, ptrs, words, 3);
-    launch_copy_u32(h->stream, T.lv[0].sflag, h->vflag, flag_words(T));
-    launch_bucket(h->stream, T, h->pa, h->n, h->S, h->CH, h->counters);
+    launch_bucket(h->stream, h->dd, h->dims);
   }
-  Physics ph = h->ph;
-  ph.dt = dt;
   {
     PhaseScope ps(h, AGIMP_PHASE_P2G);
-    launch_p2g(h->stream, T, h->pa, h->S, ph, h->p2g_grid);
+    launch_p2g(h->stream, h->dd, h->dims, dt);
   }
   {
     PhaseScope ps(h, AGIMP_PHASE_GRID);
-    launch_grid(h->stream, T, ph, h->grid_grid);
+    launch_grid(h->stream, h->dd, h->dims, dt);
   }
   {
     PhaseScope ps(h, AGIMP_PHASE_G2P);
-    launch_g2p(h->stream, T, h->pa, h->pb, h->n, h->S, ph, h->counters);
+    launch_g2p(h->stream, h->dd, h->dims, dt);
   }
-  std::swap(h->pa, h->pb);
+  h->host_parity ^= 1;
+  HIPCHK(h, hipGetLastError());
   if (h->marks.size() > 4096) {
     int rc = collect_marks(h);
     if (rc) return rc;
   }
-  h->have_slists = 1;
-  HIPCHK(h, hipGetLastError());
   return 0;
+}
+
+// capture one full substep of the scenes behind `dd` into `cap` (dt taken from the descriptors)
+hipError_t capture_substep(hipStream_t s, const SceneDesc* dd, const Dims& m, Captured& cap) {
+  cap.release();
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  launch_refine(s, dd, m);
+  launch_bucket(s, dd, m);
+  launch_p2g(s, dd, m, 0.f);
+  launch_grid(s, dd, m, 0.f);
+  launch_g2p(s, dd, m, 0.f);
+  hipError_t le = hipGetLastError();
+  e = hipStreamEndCapture(s, &cap.graph);
+  if (le != hipSuccess) return le;
+  if (e != hipSuccess) return e;
+  return hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
 }
 
 }  // namespace
@@ -243,21 +275,21 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   if (c.dim != 2 && c.dim != 3) return AGIMP_EINVAL;
   if (c.level < 1 || c.level > MAXL) return AGIMP_EINVAL;
   if (c.coarsest_size < 1 || c.n_particles < 1 || c.n_particles > (int64_t)1 << 30) return AGIMP_EINVAL;
-  if (std::fabs(c.radius - 0.5f) > 0.0f) return AGIMP_EINVAL;  // B-spline form needs r = 0.5
-  if (c.particles_per_chunk < 16 || c.particles_per_chunk > 2048) return AGIMP_EINVAL;  // P2G_MAX_CH
+  if (std::fabs(c.radius - 0.5f) > 0.0f) return AGIMP_EINVAL;  // the B-spline form needs r = 0.5
+  if (c.particles_per_chunk < AGIMP_MIN_CHUNK || c.particles_per_chunk > AGIMP_MAX_CHUNK) return AGIMP_EINVAL;
   if (c.slab_enable && (c.slab_axis < 0 || c.slab_axis >= c.dim)) return AGIMP_EINVAL;
   agimp_handle* h = new agimp_handle();
   h->cfg = c;
   h->D = c.dim;
   h->L = c.level;
   h->n = (int)c.n_particles;
-  h->CH = c.particles_per_chunk;
   int rc = 0;
   if (hipSetDevice(c.device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     delete h;
     return AGIMP_EHIP;
   }
-  Topo& T = h->T;
+  SceneDesc& sd = h->hd;
+  Topo& T = sd.T;
   T.dim = c.dim;
   T.L = c.level;
   T.c = c.coarsest_size;
@@ -291,7 +323,7 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   rc = rc ? rc : dalloc(h, &gridbuf, cells);
   rc = rc ? rc : dalloc(h, &flagbuf, (flags + 3) & ~(size_t)3);
   rc = rc ? rc : dalloc(h, &slistbuf, flags);
-  rc = rc ? rc : dalloc(h, &h->vflag, (flags + 3) / 4);
+  rc = rc ? rc : dalloc(h, &sd.vflag, (flags + 3) / 4);
   rc = rc ? rc : dalloc(h, &T.node, (size_t)T.lv[T.L - 1].nblocks * p4);
   rc = rc ? rc : dalloc(h, &T.scount, MAXL);
   rc = rc ? rc : dalloc(h, &T.cflag, (flags + 3) & ~(size_t)3);
@@ -307,16 +339,20 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
     co += (size_t)T.lv[l].nblocks * p4;
     fo += (size_t)T.lv[l].nblocks;
   }
-  // particles
+  sd.w_cells = cells / 4;
+  sd.w_node = (size_t)T.lv[T.L - 1].nblocks * p4 / 4;
+  sd.w_flags = (flags + 3) / 4;
+  // particles: two structure-of-arrays buffers
   const int NF = (c.dim == 2) ? PLayout<2>::NF : PLayout<3>::NF;
   int64_t stride = ((int64_t)h->n + 63) & ~(int64_t)63;
-  for (PBuf* pbuf : {&h->pa, &h->pb}) {
-    pbuf->stride = stride;
-    rc = rc ? rc : dalloc(h, &pbuf->f, (size_t)NF * stride);
-    rc = rc ? rc : dalloc(h, &pbuf->g, stride);
-    rc = rc ? rc : dalloc(h, &pbuf->id, stride);
+  for (int b = 0; b < 2; ++b) {
+    PBuf& pb = sd.P[b];
+    pb.stride = stride;
+    rc = rc ? rc : dalloc(h, &pb.f, (size_t)NF * stride);
+    rc = rc ? rc : dalloc(h, &pb.g, stride);
+    rc = rc ? rc : dalloc(h, &pb.id, stride);
   }
-  SortBufs& S = h->S;
+  SortBufs& S = sd.S;
   rc = rc ? rc : dalloc(h, &S.gb, stride);
   rc = rc ? rc : dalloc(h, &S.rk, stride);
   rc = rc ? rc : dalloc(h, &S.perm, stride);
@@ -327,15 +363,21 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   rc = rc ? rc : dalloc(h, &S.tiles, (T.total_blocks + 2047) / 2048 + 1);
   rc = rc ? rc : dalloc(h, &S.totals, 4);
   rc = rc ? rc : dalloc(h, &S.frozen, 4);
-  h->max_chunks = (int)std::min<int64_t>((int64_t)h->n / h->CH + std::min<int64_t>(h->n, T.total_blocks) + 1,
+  // enough chunks for the smallest chunk size the handle can be switched to
+  h->max_chunks = (int)std::min<int64_t>((int64_t)h->n / AGIMP_MIN_CHUNK + std::min<int64_t>(h->n, T.total_blocks) + 1,
                                          (int64_t)1 << 30);
   rc = rc ? rc : dalloc(h, &S.chunks, h->max_chunks);
-  rc = rc ? rc : dalloc(h, &h->counters, NCOUNTERS);
+  rc = rc ? rc : dalloc(h, &sd.cnt, NCOUNTERS);
+  rc = rc ? rc : dalloc(h, &sd.parity, 1);
+  rc = rc ? rc : dalloc(h, &h->dd, 1);
   if (rc) { agimp_destroy(h); return rc; }
-  h->ph = make_physics(c);
-  h->p2g_grid = std::max(1, std::min(h->max_chunks, 8192));
-  h->grid_grid = 4096;
-  if (hipStreamSynchronize(h->stream) != hipSuccess) { agimp_destroy(h); return AGIMP_EHIP; }
+  sd.ph = make_physics(c);
+  sd.n = h->n;
+  sd.static_mode = 0;
+  sd.CH = c.particles_per_chunk;
+  h->dims = single_dims(h);
+  rc = upload_desc(h);
+  if (rc) { agimp_destroy(h); return rc; }
   *out = h;
   return 0;
 }
@@ -343,11 +385,12 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
 int agimp_destroy(agimp_handle* h) {
   if (!h) return AGIMP_EINVAL;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  h->graph.release();
   for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
   for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
-  if (h->st_ret) (void)hipFree(h->st_ret);
-  if (h->st_act) (void)hipFree(h->st_act);
+  if (h->hd.st_ret) (void)hipFree((void*)h->hd.st_ret);
+  if (h->hd.st_act) (void)hipFree((void*)h->hd.st_act);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
   return 0;
@@ -379,10 +422,11 @@ int agimp_set_particles(agimp_handle* h, const float* x, const float* v, const f
   e = e ? e : hipMemcpyAsync(dg, g, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
   if (Jp && e == hipSuccess) e = hipMemcpyAsync(dJ, Jp, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) {
-    launch_upload(h->stream, D, h->pa, n, dx, dv, dF, dC, dm, dg, Jp ? dJ : nullptr, h->S.flv);
-    launch_stress(h->stream, D, h->pa, n, h->ph);
-    e = hipStreamSynchronize(h->stream);
+    launch_upload(h->stream, h->dd, D, n, dx, dv, dF, dC, dm, dg, Jp ? dJ : nullptr);
+    launch_stress(h->stream, h->dd, D, n);
+    e = hipGetLastError();
   }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("set_particles: ") + hipGetErrorString(e));
   h->has_particles = 1;
@@ -391,24 +435,36 @@ int agimp_set_particles(agimp_handle* h, const float* x, const float* v, const f
 
 int agimp_set_static_levels(agimp_handle* h, const int8_t* ret_level, const int8_t* act_level) {
   if (!h) return AGIMP_EINVAL;
+  h->graph.release();
   if (!ret_level || !act_level) {
-    h->static_mode = 0;
-    return 0;
+    h->hd.static_mode = 0;
+    h->dims = single_dims(h);
+    return upload_desc(h);
   }
-  size_t E = (size_t)h->T.lv[h->L - 1].E;
+  size_t E = (size_t)h->hd.T.lv[h->L - 1].E;
   size_t cnt = (h->D == 2) ? E * E : E * E * E;
   for (size_t i = 0; i < cnt; ++i)
     if (ret_level[i] < 0 || ret_level[i] >= h->L || act_level[i] < -1 || act_level[i] >= h->L)
       return fail(h, AGIMP_EINVAL, "static level map entry outside [0, level)");
-  if (!h->st_ret) {
-    HIPCHK(h, hipMalloc(&h->st_ret, cnt));
-    HIPCHK(h, hipMalloc(&h->st_act, cnt));
+  if (!h->hd.st_ret) {
+    int8_t *r = nullptr, *a = nullptr;
+    HIPCHK(h, hipMalloc(&r, cnt));
+    HIPCHK(h, hipMalloc(&a, cnt));
+    h->hd.st_ret = r;
+    h->hd.st_act = a;
   }
-  HIPCHK(h, hipMemcpyAsync(h->st_ret, ret_level, cnt, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipMemcpyAsync(h->st_act, act_level, cnt, hipMemcpyHostToDevice, h->stream));
-  HIPCHK(h, hipStreamSynchronize(h->stream));
-  h->static_mode = 1;
-  return 0;
+  HIPCHK(h, hipMemcpyAsync((void*)h->hd.st_ret, ret_level, cnt, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipMemcpyAsync((void*)h->hd.st_act, act_level, cnt, hipMemcpyHostToDevice, h->stream));
+  h->hd.static_mode = 1;
+  h->dims = single_dims(h);
+  return upload_desc(h);
+}
+
+int agimp_set_particles_per_chunk(agimp_handle* h, int32_t ch) {
+  if (!h) return AGIMP_EINVAL;
+  if (ch < AGIMP_MIN_CHUNK || ch > AGIMP_MAX_CHUNK) return fail(h, AGIMP_EINVAL, "chunk size out of range");
+  h->hd.CH = ch;
+  return upload_desc(h);
 }
 
 int agimp_refine(agimp_handle* h) {
@@ -434,10 +490,33 @@ int agimp_substep_fixed_topology(agimp_handle* h, float dt) {
 
 int agimp_advance(agimp_handle* h, float dt, int32_t n) {
   if (n < 0) return AGIMP_EINVAL;
-  for (int i = 0; i < n; ++i) {
-    int rc = agimp_substep(h, dt);
-    if (rc) return rc;
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (n == 0) return 0;
+  if (h->timing) {  // phase timing needs the eager path
+    for (int i = 0; i < n; ++i) {
+      rc = agimp_substep(h, dt);
+      if (rc) return rc;
+    }
+    return 0;
   }
+  // replay one captured substep n times; dt lives in the device descriptor
+  if (dt != h->graph_dt) {
+    h->hd.ph.dt = dt;
+    rc = upload_desc(h);
+    if (rc) return rc;
+    h->graph_dt = dt;
+  }
+  if (!h->graph.exec) {
+    hipError_t e = capture_substep(h->stream, h->dd, h->dims, h->graph);
+    if (e != hipSuccess) {
+      h->graph.release();
+      return fail(h, AGIMP_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+  }
+  for (int i = 0; i < n; ++i) HIPCHK(h, hipGraphLaunch(h->graph.exec, h->stream));
+  h->host_parity ^= (n & 1);
+  h->topo_valid = 1;
   return 0;
 }
 
@@ -456,7 +535,7 @@ int agimp_get_particles(agimp_handle* h, float* x, float* v, float* F, float* C,
   float* dJ = dC + nm;
   int* dfl = (int*)(dJ + n);
   int* dg = dfl + n;
-  launch_download(h->stream, D, h->pa, n, h->S.flv, dx, dv, dF, dC, dJ, dfl, dg);
+  launch_download(h->stream, h->dd, D, n, dx, dv, dF, dC, dJ, dfl, dg);
   hipError_t e = hipGetLastError();
   if (x && e == hipSuccess) e = hipMemcpyAsync(x, dx, nv * 4, hipMemcpyDeviceToHost, h->stream);
   if (v && e == hipSuccess) e = hipMemcpyAsync(v, dv, nv * 4, hipMemcpyDeviceToHost, h->stream);
@@ -474,13 +553,13 @@ int agimp_get_particles(agimp_handle* h, float* x, float* v, float* F, float* C,
 int agimp_get_level(agimp_handle* h, int32_t l, int32_t* cell_mask, float* m, float* v) {
   if (!h || l < 0 || l >= h->L) return AGIMP_EINVAL;
   size_t tot = 1;
-  for (int a = 0; a < h->D; ++a) tot *= (size_t)h->T.lv[l].e;
+  for (int a = 0; a < h->D; ++a) tot *= (size_t)h->hd.T.lv[l].e;
   char* tmp = nullptr;
   HIPCHK(h, hipMalloc(&tmp, tot * (4 + 4 + 4 * (size_t)h->D)));
   int* dc = (int*)tmp;
   float* dm = (float*)(dc + tot);
   float* dv = dm + tot;
-  launch_dense_level(h->stream, h->T, l, dc, dm, dv);
+  launch_dense_level(h->stream, h->dd, h->hd.T, l, dc, dm, dv);
   hipError_t e = hipGetLastError();
   if (cell_mask && e == hipSuccess) e = hipMemcpyAsync(cell_mask, dc, tot * 4, hipMemcpyDeviceToHost, h->stream);
   if (m && e == hipSuccess) e = hipMemcpyAsync(m, dm, tot * 4, hipMemcpyDeviceToHost, h->stream);
@@ -494,10 +573,10 @@ int agimp_get_level(agimp_handle* h, int32_t l, int32_t* cell_mask, float* m, fl
 int agimp_get_node_mask(agimp_handle* h, int32_t* out) {
   if (!h || !out) return AGIMP_EINVAL;
   size_t tot = 1;
-  for (int a = 0; a < h->D; ++a) tot *= (size_t)h->T.lv[h->L - 1].e;
+  for (int a = 0; a < h->D; ++a) tot *= (size_t)h->hd.T.lv[h->L - 1].e;
   int* d = nullptr;
   HIPCHK(h, hipMalloc(&d, tot * 4));
-  launch_dense_node(h->stream, h->T, d);
+  launch_dense_node(h->stream, h->dd, h->hd.T, d);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(out, d, tot * 4, hipMemcpyDeviceToHost, h->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -509,7 +588,7 @@ int agimp_get_node_mask(agimp_handle* h, int32_t* out) {
 int agimp_get_counters(agimp_handle* h, int64_t* out, int32_t n) {
   if (!h || !out || n < 0) return AGIMP_EINVAL;
   unsigned long long tmp[NCOUNTERS];
-  HIPCHK(h, hipMemcpyAsync(tmp, h->counters, sizeof(tmp), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipMemcpyAsync(tmp, h->hd.cnt, sizeof(tmp), hipMemcpyDeviceToHost, h->stream));
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (int i = 0; i < n && i < NCOUNTERS; ++i) out[i] = (int64_t)tmp[i];
   return 0;
@@ -517,14 +596,10 @@ int agimp_get_counters(agimp_handle* h, int64_t* out, int32_t n) {
 
 int agimp_reset_counters(agimp_handle* h) {
   if (!h) return AGIMP_EINVAL;
-  launch_fill_u32(h->stream, h->counters, 0, NCOUNTERS * 2);
-  HIPCHK(h, hipGetLastError());
+  unsigned long long zero[NCOUNTERS] = {};
+  HIPCHK(h, hipMemcpyAsync(h->hd.cnt, zero, sizeof(zero), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
   return 0;
-}
-
-int agimp_level_extent(agimp_handle* h, int32_t l) {
-  if (!h || l < 0 || l >= h->L) return AGIMP_EINVAL;
-  return h->T.lv[l].e;
 }
 
 int agimp_set_timing(agimp_handle* h, int32_t on) {
@@ -546,6 +621,11 @@ int agimp_get_timings(agimp_handle* h, double* ms, int64_t* calls, int32_t n) {
   return 0;
 }
 
+int agimp_level_extent(agimp_handle* h, int32_t l) {
+  if (!h || l < 0 || l >= h->L) return AGIMP_EINVAL;
+  return h->hd.T.lv[l].e;
+}
+
 int agimp_sync(agimp_handle* h) {
   if (!h) return AGIMP_EINVAL;
   HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -553,6 +633,110 @@ int agimp_sync(agimp_handle* h) {
 }
 
 const char* agimp_last_error(agimp_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+// ------------------------------------------------------------------------------ batches
+
+int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out) {
+  if (!scenes || n < 1 || !out) return AGIMP_EINVAL;
+  *out = nullptr;
+  agimp_batch* b = new agimp_batch();
+  Dims m{};
+  m.dim = scenes[0] ? scenes[0]->D : 0;
+  m.nscenes = n;
+  m.list_grid = std::max(16, 2048 / n);
+  m.grid_grid = std::max(16, 4096 / n);
+  int p2g = 0;
+  for (int i = 0; i < n; ++i) {
+    agimp_handle* h = scenes[i];
+    if (!h || h->D != m.dim || h->cfg.device != scenes[0]->cfg.device || !h->has_particles) {
+      delete b;
+      return AGIMP_EINVAL;
+    }
+    (void)hipStreamSynchronize(h->stream);
+    b->scenes.push_back(h);
+    m.maxL = std::max(m.maxL, h->L);
+    m.any_static |= h->hd.static_mode;
+    m.any_dynamic |= !h->hd.static_mode;
+    m.max_n = std::max(m.max_n, (size_t)h->n);
+    m.max_total_blocks = std::max(m.max_total_blocks, (size_t)h->hd.T.total_blocks);
+    m.max_fill_words = std::max(m.max_fill_words, h->hd.w_cells + h->hd.w_node + 2 * h->hd.w_flags);
+    p2g = std::max(p2g, h->max_chunks);
+  }
+  m.p2g_grid = std::max(16, std::min(p2g, 8192 / n));
+  b->dims = m;
+  b->device = scenes[0]->cfg.device;
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&b->dd, sizeof(SceneDesc) * (size_t)n) != hipSuccess) {
+    agimp_batch_destroy(b);
+    return AGIMP_EHIP;
+  }
+  std::vector<SceneDesc> host(n);
+  for (int i = 0; i < n; ++i) host[i] = b->scenes[i]->hd;
+  if (hipMemcpy(b->dd, host.data(), sizeof(SceneDesc) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+    agimp_batch_destroy(b);
+    return AGIMP_EHIP;
+  }
+  *out = b;
+  return 0;
+}
+
+int agimp_batch_destroy(agimp_batch* b) {
+  if (!b) return AGIMP_EINVAL;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->graph.release();
+  if (b->dd) (void)hipFree(b->dd);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int agimp_batch_set_particles_per_chunk(agimp_batch* b, const int32_t* ch) {
+  if (!b || !ch) return AGIMP_EINVAL;
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    if (ch[i] < AGIMP_MIN_CHUNK || ch[i] > AGIMP_MAX_CHUNK) return AGIMP_EINVAL;
+    b->scenes[i]->hd.CH = ch[i];
+    int rc = upload_desc(b->scenes[i]);
+    if (rc) return rc;
+    if (hipMemcpy(b->dd + i, &b->scenes[i]->hd, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess)
+      return AGIMP_EHIP;
+  }
+  return 0;
+}
+
+int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps) {
+  if (!b || !dt || steps < 0) return AGIMP_EINVAL;
+  const int n = (int)b->scenes.size();
+  // per-scene dt in the descriptors (read by the kernels at run time)
+  for (int i = 0; i < n; ++i) {
+    agimp_handle* h = b->scenes[i];
+    if (h->hd.ph.dt != dt[i]) {
+      h->hd.ph.dt = dt[i];
+      h->graph_dt = -1.f;
+      if (hipMemcpy(b->dd + i, &h->hd, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess) return AGIMP_EHIP;
+      if (hipMemcpy(h->dd, &h->hd, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess) return AGIMP_EHIP;
+    }
+  }
+  if (!b->graph.exec) {
+    hipError_t e = capture_substep(b->stream, b->dd, b->dims, b->graph);
+    if (e != hipSuccess) {
+      b->graph.release();
+      b->err = std::string("graph capture: ") + hipGetErrorString(e);
+      return AGIMP_EHIP;
+    }
+  }
+  for (int s = 0; s < steps; ++s)
+    if (hipGraphLaunch(b->graph.exec, b->stream) != hipSuccess) return AGIMP_EHIP;
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return AGIMP_EHIP;
+  for (agimp_handle* h : b->scenes) {
+    h->host_parity ^= (steps & 1);
+    h->topo_valid = 1;
+  }
+  return 0;
+}
+
+const char* agimp_batch_last_error(agimp_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+// ------------------------------------------------------------------------------ host maths
 
 int agimp_host_svd(int32_t dim, const double* F, double* U, double* s, double* V) {
   if (!F || !U || !s || !V) return AGIMP_EINVAL;

@@ -84,6 +84,41 @@ struct PBuf {
   int64_t stride;
 };
 
+// scratch of the (level, leaf block) counting sort and the P2G chunk list
+struct SortBufs {
+  int* gb;       // [n]  global block id of each slot's particle (-1: frozen)
+  int* rk;       // [n]  rank inside its block (or among frozen particles)
+  int* perm;     // [n]  sorted position -> slot
+  int* flv;      // [n]  fl_p of the particle now in each slot
+  int* bcnt;     // [total_blocks] particles per block
+  int* boff;     // [total_blocks] first sorted position of each block
+  int* choff;    // [total_blocks] first chunk of each block
+  int2* tiles;   // scan tile partials
+  int* totals;   // [2] active particles, chunks
+  int* frozen;   // [1] frozen-particle counter
+  int4* chunks;  // (gbid, first position, count, -)
+};
+
+// Everything the kernels need about one scene.  Kernels take an array of these and pick
+// theirs with blockIdx.y, so one launch advances a whole batch of scenes; a single
+// simulator is a batch of one.  The current particle buffer is P[*parity] (device-side, so
+// that a captured launch sequence can be replayed).
+struct SceneDesc {
+  Topo T;
+  PBuf P[2];
+  SortBufs S;
+  Physics ph;
+  int n;
+  int static_mode;
+  int CH;  // P2G particles per work chunk
+  const int8_t* st_ret;
+  const int8_t* st_act;
+  unsigned long long* cnt;
+  uint32_t* vflag;
+  int* parity;
+  size_t w_cells, w_node, w_flags;  // 32-bit words of the cell masks, node mask, block flags
+};
+
 // counters kept on the device (int64 each)
 enum Counter : int {
   CNT_OOB = 0,        // accesses outside the padded level boxes (reference: undefined behaviour)

@@ -31,6 +31,13 @@ namespace agimp {
 template <int D>
 struct Pow4 { static constexpr int v = (D == 2) ? 16 : 64; };
 
+// the scene of this workgroup (blockIdx.y) and its current particle buffer
+#define AGIMP_SCENE                          \
+  const SceneDesc& sd = Dsc[blockIdx.y];     \
+  const Topo& T = sd.T;                      \
+  const int par = *sd.parity;                \
+  (void)par;
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   size_t st = (size_t)gridDim.x * blockDim.x;
@@ -115,14 +122,14 @@ __device__ __forceinline__ void mark_valid_node(const Topo& T, const int* f) {
   }
 }
 
-// zero several buffers in one launch
+// zero the per-substep buffers of every scene in one launch
 struct FillRanges {
   uint32_t* p[8];
   size_t n[8];
   int k;
 };
 
-__global__ void k_fill_ranges(FillRanges R) {
+__device__ __forceinline__ void fill_zero(const FillRanges& R) {
   size_t total = 0;
   for (int r = 0; r < R.k; ++r) total += R.n[r];
   size_t st = (size_t)gridDim.x * blockDim.x;
@@ -134,8 +141,49 @@ __global__ void k_fill_ranges(FillRanges R) {
   }
 }
 
+// before refine: cell masks, node mask, block flags, list counters
+__global__ void k_clear_refine(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  FillRanges R{{(uint32_t*)T.lv[0].cell, (uint32_t*)T.node, (uint32_t*)T.lv[0].sflag, (uint32_t*)T.cflag,
+                (uint32_t*)T.scount, (uint32_t*)T.ccount},
+               {sd.w_cells, sd.w_node, sd.w_flags, sd.w_flags, (size_t)MAXL, 4},
+               6};
+  fill_zero(R);
+}
+
+// before bucketing: block counters, the frozen counter, grid-phase list counters
+__global__ void k_clear_bucket(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  FillRanges R{{(uint32_t*)sd.S.bcnt, (uint32_t*)sd.S.frozen, (uint32_t*)T.scount},
+               {(size_t)T.total_blocks, 1, (size_t)MAXL},
+               3};
+  fill_zero(R);
+}
+
+// the valid-node block flags of the octree: save after refine (to_saved) or restore
+__global__ void k_copy_flags(const SceneDesc* Dsc, int to_saved) {
+  AGIMP_SCENE
+  uint32_t* a = (uint32_t*)T.lv[0].sflag;
+  uint32_t* v = sd.vflag;
+  size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < sd.w_flags; i += st) {
+    if (to_saved) v[i] = a[i];
+    else a[i] = v[i];
+  }
+}
+
+// flip the current particle buffer (end of a physics step)
+__global__ void k_flip(const SceneDesc* Dsc) {
+  if (threadIdx.x == 0) *Dsc[blockIdx.y].parity ^= 1;
+}
+
 // append the ids of set flags to list (order unspecified)
-__global__ void k_compact_flags(const uint8_t* flags, int n, int* list, int* count) {
+__global__ void k_compact_cells(const SceneDesc* Dsc, int which) {
+  AGIMP_SCENE
+  const uint8_t* flags = T.cflag;
+  const int n = T.total_blocks;
+  int* list = T.clist;
+  int* count = T.ccount + which;
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   bool on = g < n && flags[g] != 0;
   int idx = wave_run_atomic_add(count, on);
@@ -144,7 +192,8 @@ __global__ void k_compact_flags(const uint8_t* flags, int n, int* list, int* cou
 
 // ------------------------------------------------------------------------------ clear
 template <int D>
-__global__ void k_clear_sgrid(Topo T) {
+__global__ void k_clear_sgrid(const SceneDesc* Dsc) {
+  AGIMP_SCENE
   size_t total = 0;
   for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
   size_t st = (size_t)gridDim.x * blockDim.x;
@@ -166,7 +215,12 @@ __global__ void k_clear_sgrid(Topo T) {
 // ------------------------------------------------------------------------------ activation
 // adaptive_gimp.py:87-96: OR ACTIVATED into cell_mask[g_p][base + dI], dI in {0,1}^d
 template <int D>
-__global__ void k_activate_dynamic(Topo T, PBuf P, int n, unsigned long long* cnt) {
+__global__ void k_activate_dynamic(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  if (sd.static_mode) return;
+  const PBuf& P = sd.P[par];
+  const int n = sd.n;
+  unsigned long long* cnt = sd.cnt;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int g = P.g[k];
@@ -216,8 +270,14 @@ __device__ __forceinline__ int static_gi(const Topo& T, const int8_t* ret, const
 }
 
 template <int D>
-__global__ void k_activate_static(Topo T, PBuf P, int n, const int8_t* ret, const int8_t* act,
-                                  unsigned long long* cnt) {
+__global__ void k_activate_static(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  if (!sd.static_mode) return;
+  const PBuf& P = sd.P[par];
+  const int n = sd.n;
+  const int8_t* ret = sd.st_ret;
+  const int8_t* act = sd.st_act;
+  unsigned long long* cnt = sd.cnt;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const LevelDesc& F = T.lv[T.L - 1];
@@ -252,7 +312,12 @@ __global__ void k_activate_static(Topo T, PBuf P, int n, const int8_t* ret, cons
 // ------------------------------------------------------------------------------ culling
 // adaptive_gimp.py:98-109 for all l >= 1 at once (ACTIVATED bits do not change here)
 template <int D>
-__global__ void k_mark_hier(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+__global__ void k_mark_hier(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  if (sd.static_mode) return;
+  const int* list = T.clist;
+  const int* countp = T.ccount + 0;
+  unsigned long long* cnt = sd.cnt;
   const int nlist = *countp;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
     int l, I[3];
@@ -285,7 +350,12 @@ __global__ void k_mark_hier(Topo T, const int* list, const int* countp, unsigned
 // adaptive_gimp.py:111-118 for all l at once: children of CULLING cells get ACTIVATED.
 // The culled cell itself keeps A|C (it compares unequal to ACTIVATED from here on).
 template <int D>
-__global__ void k_split(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+__global__ void k_split(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  if (sd.static_mode) return;
+  const int* list = T.clist;
+  const int* countp = T.ccount + 1;
+  unsigned long long* cnt = sd.cnt;
   const int nlist = *countp;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
     int l, I[3];
@@ -308,7 +378,11 @@ __global__ void k_split(Topo T, const int* list, const int* countp, unsigned lon
 // ------------------------------------------------------------------------------ node marks
 // adaptive_gimp.py:142-153: corners of exactly-ACTIVATED cells get ACTIVATED
 template <int D>
-__global__ void k_node_A(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+__global__ void k_node_A(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const int* list = T.clist;
+  const int* countp = T.ccount + 2;
+  unsigned long long* cnt = sd.cnt;
   const int nlist = *countp;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
@@ -354,7 +428,11 @@ __device__ __forceinline__ int face_parent(const Topo& T, int l, const int* I, i
 
 // adaptive_gimp.py:170-174 for all levels at once
 template <int D>
-__global__ void k_mark_T(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+__global__ void k_mark_T(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const int* list = T.clist;
+  const int* countp = T.ccount + 2;
+  unsigned long long* cnt = sd.cnt;
   const int nlist = *countp;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
@@ -393,7 +471,11 @@ __global__ void k_mark_T(Topo T, const int* list, const int* countp, unsigned lo
 
 // adaptive_gimp.py:176-183 for all levels at once, after every T mark
 template <int D>
-__global__ void k_mark_G(Topo T, const int* list, const int* countp, unsigned long long* cnt) {
+__global__ void k_mark_G(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const int* list = T.clist;
+  const int* countp = T.ccount + 2;
+  unsigned long long* cnt = sd.cnt;
   const int nlist = *countp;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
@@ -432,8 +514,16 @@ __global__ void k_mark_G(Topo T, const int* list, const int* countp, unsigned lo
 // adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
 // particle's 2^d cells; rank inside its (level, leaf block) bucket by run-aggregated atomics.
 template <int D>
-__global__ void k_levelmap(Topo T, PBuf P, int n, int* gb, int* rk, int* bcnt, int* frozen,
-                           unsigned long long* cnt) {
+__global__ void k_levelmap(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const PBuf& P = sd.P[par];
+  const int n = sd.n;
+  int* gb = sd.S.gb;
+  int* rk = sd.S.rk;
+  int* bcnt = sd.S.bcnt;
+  int* frozen = sd.S.frozen;
+  unsigned long long* cnt = sd.cnt;
+  if (blockIdx.x * blockDim.x >= (unsigned)n) return;  // whole workgroup past this scene
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = k < n;
   int fl = -1, gbid = -1;
@@ -511,7 +601,12 @@ __device__ __forceinline__ int2 load_cc(const int* bcnt, int i, int n, int CH) {
   return make_int2(c, (c + CH - 1) / CH);
 }
 
-__global__ void k_scan_reduce(const int* bcnt, int n, int CH, int2* tile_sum) {
+__global__ void k_scan_reduce(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const int* bcnt = sd.S.bcnt;
+  const int n = T.total_blocks, CH = sd.CH;
+  int2* tile_sum = sd.S.tiles;
+  if ((int)blockIdx.x * SCAN_TILE >= n) return;
   __shared__ int2 sh[SCAN_T];
   int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
   int2 s = make_int2(0, 0);
@@ -522,7 +617,11 @@ __global__ void k_scan_reduce(const int* bcnt, int n, int CH, int2* tile_sum) {
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_tiles(int2* tile_sum, int ntiles, int* totals) {
+__global__ void k_scan_tiles(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  int2* tile_sum = sd.S.tiles;
+  const int ntiles = (T.total_blocks + SCAN_TILE - 1) / SCAN_TILE;
+  int* totals = sd.S.totals;
   __shared__ int2 sh[SCAN_T];
   int2 carry = make_int2(0, 0);
   for (int base = 0; base < ntiles; base += SCAN_T) {
@@ -536,7 +635,14 @@ __global__ void k_scan_tiles(int2* tile_sum, int ntiles, int* totals) {
   if (threadIdx.x == 0) { totals[0] = carry.x; totals[1] = carry.y; }
 }
 
-__global__ void k_scan_final(const int* bcnt, int n, int CH, const int2* tile_off, int* boff, int* choff) {
+__global__ void k_scan_final(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const int* bcnt = sd.S.bcnt;
+  const int n = T.total_blocks, CH = sd.CH;
+  const int2* tile_off = sd.S.tiles;
+  int* boff = sd.S.boff;
+  int* choff = sd.S.choff;
+  if ((int)blockIdx.x * SCAN_TILE >= n) return;
   __shared__ int2 sh[SCAN_T];
   int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
   int2 v[SCAN_I];
@@ -552,8 +658,15 @@ __global__ void k_scan_final(const int* bcnt, int n, int CH, const int2* tile_of
   }
 }
 
-__global__ void k_scatter(int n, const int* gb, const int* rk, const int* boff, const int* totals, int* perm,
-                          unsigned long long* cnt) {
+__global__ void k_scatter(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const int n = sd.n;
+  const int* gb = sd.S.gb;
+  const int* rk = sd.S.rk;
+  const int* boff = sd.S.boff;
+  const int* totals = sd.S.totals;
+  int* perm = sd.S.perm;
+  unsigned long long* cnt = sd.cnt;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   int g = gb[k];
@@ -568,7 +681,13 @@ __global__ void k_scatter(int n, const int* gb, const int* rk, const int* boff, 
 
 // per occupied leaf block: P2G chunk descriptors + the blocks its 3^d stencils touch
 template <int D>
-__global__ void k_blocks(Topo T, const int* bcnt, const int* boff, const int* choff, int4* chunks, int CH) {
+__global__ void k_blocks(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const int* bcnt = sd.S.bcnt;
+  const int* boff = sd.S.boff;
+  const int* choff = sd.S.choff;
+  int4* chunks = sd.S.chunks;
+  const int CH = sd.CH;
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= T.total_blocks) return;
   int c = bcnt[g];
@@ -595,7 +714,9 @@ __global__ void k_blocks(Topo T, const int* bcnt, const int* boff, const int* ch
 
 // compact the grid-phase block flags of every level into per-level lists
 template <int D>
-__global__ void k_scompact(Topo T) {
+__global__ void k_scompact(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  if (blockIdx.x * blockDim.x >= (unsigned)T.total_blocks) return;  // whole workgroup past this scene
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = g < T.total_blocks;
   int l = 0, b = 0;
@@ -675,8 +796,14 @@ constexpr int P2G_MAX_CH = 2048; // largest work chunk (particles of one leaf bl
 //     tile (no two lanes of one instruction hit the same LDS word);
 //  3. the tile is flushed with one float atomic per non-zero node component.
 template <int D>
-__global__ void __launch_bounds__(P2G_T) k_p2g(Topo T, PBuf P, int* perm, const int4* chunks, const int* totals,
-                                               Physics ph) {
+__global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* Dsc, float dt_arg) {
+  AGIMP_SCENE
+  const PBuf& P = sd.P[par];
+  int* perm = sd.S.perm;
+  const int4* chunks = sd.S.chunks;
+  const int* totals = sd.S.totals;
+  Physics ph = sd.ph;
+  if (dt_arg > 0.f) ph.dt = dt_arg;
   using PL = PLayout<D>;
   constexpr int TS = 6;
   constexpr int TN = (D == 2) ? TS * TS : TS * TS * TS;
@@ -895,7 +1022,9 @@ __device__ __forceinline__ bool in_range_nodes(const int* I, int n) {
 // adaptive_gimp.py:289-301 as a gather onto each valid coarse node I of level l-1:
 // sources are fine nodes 2I - d with weight 2^-|d|_1.
 template <int D>
-__global__ void k_restrict(Topo T, int l) {
+__global__ void k_restrict(const SceneDesc* Dsc, int l) {
+  AGIMP_SCENE
+  if (l >= T.L) return;
   const LevelDesc& Lc = T.lv[l - 1];
   const LevelDesc& Lf = T.lv[l];
   int total = T.scount[l - 1] * Pow4<D>::v;
@@ -931,7 +1060,10 @@ __global__ void k_restrict(Topo T, int l) {
 
 // adaptive_gimp.py:277-287 on every level's list (+ sticky walls / slab collider)
 template <int D>
-__global__ void k_gridop(Topo T, Physics ph) {
+__global__ void k_gridop(const SceneDesc* Dsc, float dt_arg) {
+  AGIMP_SCENE
+  Physics ph = sd.ph;
+  if (dt_arg > 0.f) ph.dt = dt_arg;
   int tot[MAXL];
   int total = 0;
   for (int l = 0; l < T.L; ++l) { tot[l] = T.scount[l] * Pow4<D>::v; total += tot[l]; }
@@ -980,7 +1112,9 @@ __global__ void k_gridop(Topo T, Physics ph) {
 // adaptive_gimp.py:303-321 as a gather onto each level-l node J that is T, ghost or
 // all-even: J = 2I + d over valid coarse nodes I, weight 2^-|d|_1 (only d = 0 unless T/G).
 template <int D>
-__global__ void k_prolong(Topo T, int l) {
+__global__ void k_prolong(const SceneDesc* Dsc, int l) {
+  AGIMP_SCENE
+  if (l >= T.L) return;
   const LevelDesc& Lf = T.lv[l];
   const LevelDesc& Lc = T.lv[l - 1];
   int total = T.scount[l] * Pow4<D>::v;
@@ -1028,8 +1162,17 @@ __global__ void k_prolong(Topo T, int l) {
 // adaptive_gimp.py:252-275 (+ plasticity and the next substep's stress); reads slot
 // perm[pos] of the current buffer, writes slot pos of the next buffer (re-sorting for free).
 template <int D>
-__global__ void __launch_bounds__(256) k_g2p(Topo T, PBuf Pin, PBuf Pout, const int* perm, const int* gb,
-                                             int* flv, int n, Physics ph, unsigned long long* cnt) {
+__global__ void __launch_bounds__(256) k_g2p(const SceneDesc* Dsc, float dt_arg) {
+  AGIMP_SCENE
+  const PBuf& Pin = sd.P[par];
+  const PBuf& Pout = sd.P[par ^ 1];
+  const int* perm = sd.S.perm;
+  const int* gb = sd.S.gb;
+  int* flv = sd.S.flv;
+  const int n = sd.n;
+  unsigned long long* cnt = sd.cnt;
+  Physics ph = sd.ph;
+  if (dt_arg > 0.f) ph.dt = dt_arg;
   using PL = PLayout<D>;
   int pos = blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= n) return;
@@ -1138,7 +1281,11 @@ __global__ void __launch_bounds__(256) k_g2p(Topo T, PBuf Pin, PBuf Pout, const 
 
 // Kirchhoff stress of the stored F (no plastic projection): after the caller sets state
 template <int D>
-__global__ void k_stress(PBuf P, int n, Physics ph) {
+__global__ void k_stress(const SceneDesc* Dsc) {
+  AGIMP_SCENE
+  const PBuf& P = sd.P[par];
+  const int n = sd.n;
+  const Physics& ph = sd.ph;
   using PL = PLayout<D>;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -1158,8 +1305,12 @@ __global__ void k_stress(PBuf P, int n, Physics ph) {
 // ------------------------------------------------------------------------------ host I/O
 // caller order (AoS, like the reference's to_numpy layout) -> slot k = original index
 template <int D>
-__global__ void k_upload(PBuf P, int n, const float* x, const float* v, const float* F, const float* C,
-                         const float* m, const int* g, const float* Jp, int* flv) {
+__global__ void k_upload(const SceneDesc* Dsc, const float* x, const float* v, const float* F, const float* C,
+                         const float* m, const int* g, const float* Jp) {
+  AGIMP_SCENE
+  const PBuf& P = sd.P[par];
+  const int n = sd.n;
+  int* flv = sd.S.flv;
   using PL = PLayout<D>;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -1182,8 +1333,12 @@ __global__ void k_upload(PBuf P, int n, const float* x, const float* v, const fl
 
 // slot order -> caller order (AoS)
 template <int D>
-__global__ void k_download(PBuf P, int n, const int* flv, float* x, float* v, float* F, float* C, float* Jp,
-                           int* fl, int* g) {
+__global__ void k_download(const SceneDesc* Dsc, float* x, float* v, float* F, float* C, float* Jp, int* fl,
+                           int* g) {
+  AGIMP_SCENE
+  const PBuf& P = sd.P[par];
+  const int n = sd.n;
+  const int* flv = sd.S.flv;
   using PL = PLayout<D>;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -1206,7 +1361,8 @@ __global__ void k_download(PBuf P, int n, const int* flv, float* x, float* v, fl
 // dense readback of one level over [0, e)^d; culled cells (A|C kept for speed) read as the
 // reference's 0 (or GHOST when a ghost mark landed on them later)
 template <int D>
-__global__ void k_dense_level(Topo T, int l, int* cell, float* m, float* v) {
+__global__ void k_dense_level(const SceneDesc* Dsc, int l, int* cell, float* m, float* v) {
+  AGIMP_SCENE
   const LevelDesc& Ll = T.lv[l];
   size_t tot = 1;
   for (int a = 0; a < D; ++a) tot *= (size_t)Ll.e;
@@ -1227,7 +1383,8 @@ __global__ void k_dense_level(Topo T, int l, int* cell, float* m, float* v) {
 }
 
 template <int D>
-__global__ void k_dense_node(Topo T, int* out) {
+__global__ void k_dense_node(const SceneDesc* Dsc, int* out) {
+  AGIMP_SCENE
   const LevelDesc& Ll = T.lv[T.L - 1];
   size_t tot = 1;
   for (int a = 0; a < D; ++a) tot *= (size_t)Ll.e;
@@ -1252,130 +1409,101 @@ size_t cells_total(const Topo& T, int lmin, int lmax) {
   return s;
 }
 
-void launch_fill_u32(hipStream_t s, void* p, uint32_t v, size_t n) {
-  if (n == 0) return;
-  unsigned g = (unsigned)std::min<size_t>(blocks_for(n, 256), 65535u * 4);
-  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, s, (uint32_t*)p, v, n);
+static inline dim3 grid2(size_t x, int nscenes) {
+  return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(x, 1u << 20)), (unsigned)nscenes);
 }
 
-void launch_copy_u32(hipStream_t s, void* dst, const void* src, size_t n) {
-  if (n == 0) return;
-  unsigned g = (unsigned)std::min<size_t>(blocks_for(n, 256), 65535u * 4);
-  hipLaunchKernelGGL(k_copy_u32, dim3(g), dim3(256), 0, s, (uint32_t*)dst, (const uint32_t*)src, n);
-}
-
-void launch_clear_sgrid(hipStream_t s, const Topo& T, int grid_blocks) {
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_clear_sgrid<DD>, dim3(grid_blocks), dim3(256), 0, s, T));
-}
-
-void launch_fill_ranges(hipStream_t s, void* const* ptrs, const size_t* words, int k) {
-  FillRanges R{};
-  size_t total = 0;
-  R.k = 0;
-  for (int i = 0; i < k && i < 8; ++i) {
-    if (words[i] == 0) continue;
-    R.p[R.k] = (uint32_t*)ptrs[i];
-    R.n[R.k] = words[i];
-    total += words[i];
-    ++R.k;
-  }
-  if (total == 0) return;
-  unsigned g = (unsigned)std::min<size_t>(blocks_for(total, 256), 8192);
-  hipLaunchKernelGGL(k_fill_ranges, dim3(g), dim3(256), 0, s, R);
-}
-
-static void compact_cells(hipStream_t s, const Topo& T, int which) {
-  hipLaunchKernelGGL(k_compact_flags, dim3(blocks_for(T.total_blocks, 256)), dim3(256), 0, s, T.cflag,
-                     T.total_blocks, T.clist, T.ccount + which);
-}
-
-void launch_refine(hipStream_t s, const Topo& T, const PBuf& P, int n, int static_mode, const int8_t* ret,
-                   const int8_t* act, unsigned long long* cnt) {
-  unsigned pb = blocks_for(n, 256);
-  const unsigned lg = 2048;  // grid-stride launches over block lists of device-side length
-  AGIMP_DIM(T.dim, {
-    if (static_mode) {
-      hipLaunchKernelGGL(k_activate_static<DD>, dim3(pb), dim3(256), 0, s, T, P, n, ret, act, cnt);
-    } else {
-      hipLaunchKernelGGL(k_activate_dynamic<DD>, dim3(pb), dim3(256), 0, s, T, P, n, cnt);
-      if (T.L > 1) {
-        compact_cells(s, T, 0);
-        hipLaunchKernelGGL(k_mark_hier<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 0, cnt);
-        compact_cells(s, T, 1);
-        hipLaunchKernelGGL(k_split<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 1, cnt);
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m) {
+  const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
+  const dim3 bg = grid2(blocks_for(m.max_total_blocks, 256), m.nscenes);
+  const dim3 lg = grid2(m.list_grid, m.nscenes);
+  const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
+  AGIMP_DIM(m.dim, {
+    hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_clear_refine, fg, dim3(256), 0, s, d);
+    if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d);
+    if (m.any_dynamic) {
+      hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d);
+      if (m.maxL > 1) {
+        hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 0);
+        hipLaunchKernelGGL(k_mark_hier<DD>, lg, dim3(256), 0, s, d);
+        hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 1);
+        hipLaunchKernelGGL(k_split<DD>, lg, dim3(256), 0, s, d);
       }
     }
-    compact_cells(s, T, 2);
-    hipLaunchKernelGGL(k_node_A<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 2, cnt);
-    hipLaunchKernelGGL(k_mark_T<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 2, cnt);
-    hipLaunchKernelGGL(k_mark_G<DD>, dim3(lg), dim3(256), 0, s, T, T.clist, T.ccount + 2, cnt);
+    hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 2);
+    hipLaunchKernelGGL(k_node_A<DD>, lg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_mark_T<DD>, lg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_mark_G<DD>, lg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 1);
   });
 }
 
-void launch_bucket(hipStream_t s, const Topo& T, const PBuf& P, int n, SortBufs& S, int CH, unsigned long long* cnt) {
-  unsigned pb = blocks_for(n, 256);
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_levelmap<DD>, dim3(pb), dim3(256), 0, s, T, P, n, S.gb, S.rk, S.bcnt,
-                                      S.frozen, cnt));
-  int nb = T.total_blocks;
-  int ntiles = (nb + SCAN_TILE - 1) / SCAN_TILE;
-  hipLaunchKernelGGL(k_scan_reduce, dim3(ntiles), dim3(SCAN_T), 0, s, S.bcnt, nb, CH, S.tiles);
-  hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(SCAN_T), 0, s, S.tiles, ntiles, S.totals);
-  hipLaunchKernelGGL(k_scan_final, dim3(ntiles), dim3(SCAN_T), 0, s, S.bcnt, nb, CH, S.tiles, S.boff, S.choff);
-  hipLaunchKernelGGL(k_scatter, dim3(pb), dim3(256), 0, s, n, S.gb, S.rk, S.boff, S.totals, S.perm, cnt);
-  AGIMP_DIM(T.dim, {
-    hipLaunchKernelGGL(k_blocks<DD>, dim3(blocks_for(nb, 256)), dim3(256), 0, s, T, S.bcnt, S.boff, S.choff,
-                       S.chunks, CH);
-    hipLaunchKernelGGL(k_scompact<DD>, dim3(blocks_for(nb, 256)), dim3(256), 0, s, T);
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m) {
+  const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
+  const dim3 bg = grid2(blocks_for(m.max_total_blocks, 256), m.nscenes);
+  const dim3 lg = grid2(m.list_grid, m.nscenes);
+  const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
+  const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
+  AGIMP_DIM(m.dim, {
+    hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_clear_bucket, fg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 0);
+    hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
+    hipLaunchKernelGGL(k_scan_tiles, grid2(1, m.nscenes), dim3(SCAN_T), 0, s, d);
+    hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);
+    hipLaunchKernelGGL(k_scatter, pg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_blocks<DD>, bg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_scompact<DD>, bg, dim3(256), 0, s, d);
   });
 }
 
-void launch_p2g(hipStream_t s, const Topo& T, const PBuf& Pin, SortBufs& S, const Physics& ph, int p2g_grid) {
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_p2g<DD>, dim3(p2g_grid), dim3(P2G_T), 0, s, T, Pin, S.perm, S.chunks, S.totals,
-                                      ph));
+void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
+  AGIMP_DIM(m.dim, hipLaunchKernelGGL(k_p2g<DD>, grid2(m.p2g_grid, m.nscenes), dim3(P2G_T), 0, s, d, dt));
 }
 
-void launch_grid(hipStream_t s, const Topo& T, const Physics& ph, int grid_grid) {
-  AGIMP_DIM(T.dim, {
-    for (int l = T.L - 1; l >= 1; --l)
-      hipLaunchKernelGGL(k_restrict<DD>, dim3(grid_grid), dim3(256), 0, s, T, l);
-    hipLaunchKernelGGL(k_gridop<DD>, dim3(grid_grid), dim3(256), 0, s, T, ph);
-    for (int l = 1; l < T.L; ++l) hipLaunchKernelGGL(k_prolong<DD>, dim3(grid_grid), dim3(256), 0, s, T, l);
+void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
+  const dim3 gg = grid2(m.grid_grid, m.nscenes);
+  AGIMP_DIM(m.dim, {
+    for (int l = m.maxL - 1; l >= 1; --l) hipLaunchKernelGGL(k_restrict<DD>, gg, dim3(256), 0, s, d, l);
+    hipLaunchKernelGGL(k_gridop<DD>, gg, dim3(256), 0, s, d, dt);
+    for (int l = 1; l < m.maxL; ++l) hipLaunchKernelGGL(k_prolong<DD>, gg, dim3(256), 0, s, d, l);
   });
 }
 
-void launch_g2p(hipStream_t s, const Topo& T, const PBuf& Pin, const PBuf& Pout, int n, SortBufs& S,
-                const Physics& ph, unsigned long long* cnt) {
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_g2p<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, T, Pin, Pout, S.perm,
-                                      S.gb, S.flv, n, ph, cnt));
+void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
+  AGIMP_DIM(m.dim, hipLaunchKernelGGL(k_g2p<DD>, grid2(blocks_for(m.max_n, 256), m.nscenes), dim3(256), 0, s, d, dt));
+  hipLaunchKernelGGL(k_flip, grid2(1, m.nscenes), dim3(64), 0, s, d);
 }
 
-void launch_stress(hipStream_t s, int dim, const PBuf& P, int n, const Physics& ph) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_stress<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, n, ph));
+void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_stress<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d));
 }
 
-void launch_upload(hipStream_t s, int dim, const PBuf& P, int n, const float* x, const float* v, const float* F,
-                   const float* C, const float* m, const int* g, const float* Jp, int* flv) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_upload<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, n, x, v, F, C, m, g,
-                                    Jp, flv));
+void launch_upload(hipStream_t s, const SceneDesc* d, int dim, int n, const float* x, const float* v, const float* F,
+                   const float* C, const float* m, const int* g, const float* Jp) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_upload<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, x, v, F, C, m, g,
+                                    Jp));
 }
 
-void launch_download(hipStream_t s, int dim, const PBuf& P, int n, const int* flv, float* x, float* v, float* F,
-                     float* C, float* Jp, int* fl, int* g) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_download<DD>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, n, flv, x, v, F,
-                                    C, Jp, fl, g));
+void launch_download(hipStream_t s, const SceneDesc* d, int dim, int n, float* x, float* v, float* F, float* C,
+                     float* Jp, int* fl, int* g) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_download<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, x, v, F, C, Jp,
+                                    fl, g));
 }
 
-void launch_dense_level(hipStream_t s, const Topo& T, int l, int* cell, float* m, float* v) {
+void launch_dense_level(hipStream_t s, const SceneDesc* d, const Topo& T, int l, int* cell, float* m, float* v) {
   size_t tot = 1;
   for (int a = 0; a < T.dim; ++a) tot *= (size_t)T.lv[l].e;
-  AGIMP_DIM(T.dim,
-            hipLaunchKernelGGL(k_dense_level<DD>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, T, l, cell, m, v));
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_dense_level<DD>, grid2(blocks_for(tot, 256), 1), dim3(256), 0, s, d, l, cell,
+                                      m, v));
 }
 
-void launch_dense_node(hipStream_t s, const Topo& T, int* out) {
+void launch_dense_node(hipStream_t s, const SceneDesc* d, const Topo& T, int* out) {
   size_t tot = 1;
   for (int a = 0; a < T.dim; ++a) tot *= (size_t)T.lv[T.L - 1].e;
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_dense_node<DD>, dim3(blocks_for(tot, 256)), dim3(256), 0, s, T, out));
+  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_dense_node<DD>, grid2(blocks_for(tot, 256), 1), dim3(256), 0, s, d, out));
 }
 
 }  // namespace agimp

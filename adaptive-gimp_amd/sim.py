@@ -262,6 +262,10 @@ class AdaptiveGIMP:
         _lib.check(self._L.agimp_advance(self._h, float(dt0), int(n)), self._h, "agimp_advance")
         self._invalidate()
 
+    def set_particles_per_chunk(self, ch):
+        """P2G work chunk size (the autotuned parameter)."""
+        _lib.check(self._L.agimp_set_particles_per_chunk(self._h, int(ch)), self._h, "set_particles_per_chunk")
+
     def sync(self):
         _lib.check(self._L.agimp_sync(self._h), self._h, "agimp_sync")
 
@@ -306,3 +310,50 @@ class AdaptiveGIMP:
 
     def is_ghost(self, l, I):        # :139-140
         return (self._nm(l, I) & (ACTIVATED | GHOST | T_JUNCTION)) == GHOST
+
+
+class Batch:
+    """Many independent simulators of one dimension advanced together: each substep is one
+    launch per phase for all scenes, captured once in a hipGraph and replayed.  The member
+    simulators keep their state and API (read fields after ``advance``)."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        if not self.sims:
+            raise ValueError("empty batch")
+        self._L = _lib.load()
+        for s in self.sims:
+            s._push()
+        arr = (ctypes.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        b = ctypes.c_void_p()
+        rc = self._L.agimp_batch_create(arr, len(self.sims), ctypes.byref(b))
+        if rc != 0:
+            raise _lib.AgimpError(f"agimp_batch_create failed with code {rc}")
+        self._b = b
+
+    def set_particles_per_chunk(self, ch):
+        ch = np.ascontiguousarray(np.broadcast_to(np.asarray(ch, np.int32), (len(self.sims),)))
+        rc = self._L.agimp_batch_set_particles_per_chunk(self._b, _lib.ptr(ch, ctypes.c_int32))
+        if rc != 0:
+            raise _lib.AgimpError(f"agimp_batch_set_particles_per_chunk failed with code {rc}")
+
+    def advance(self, dt, steps):
+        dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (len(self.sims),)))
+        rc = self._L.agimp_batch_advance(self._b, _lib.ptr(dt, ctypes.c_float), int(steps))
+        if rc != 0:
+            raw = self._L.agimp_batch_last_error(self._b)
+            raise _lib.AgimpError(f"agimp_batch_advance failed with code {rc}: {raw.decode() if raw else ''}")
+        for s in self.sims:
+            s._invalidate()
+
+    def close(self):
+        b = getattr(self, "_b", None)
+        if b:
+            self._L.agimp_batch_destroy(b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

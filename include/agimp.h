@@ -32,6 +32,10 @@ extern "C" {
 #define AGIMP_ENOMEM (-4)   /* device allocation failed */
 
 typedef struct agimp_handle agimp_handle;
+typedef struct agimp_batch agimp_batch;
+
+#define AGIMP_MIN_CHUNK 64    /* particles per P2G work chunk: smallest */
+#define AGIMP_MAX_CHUNK 2048  /* ... and largest */
 
 /* Simulator configuration.  Defaults (agimp_config_init) are the reference constants:
  * adaptive_gimp.py:18 (boundary_gap 4), :48-53 (radius 0.5, E 5e3, nu 0.2, gravity -100 on
@@ -54,7 +58,7 @@ typedef struct agimp_config {
   int32_t slab_enable;     /* sticky slab collider on grid nodes */
   int32_t slab_axis;
   float slab_lo, slab_hi;
-  int32_t particles_per_chunk; /* P2G work chunk (particles of one leaf block), 16..2048 */
+  int32_t particles_per_chunk; /* P2G work chunk (particles of one leaf block), AGIMP_MIN/MAX_CHUNK */
   int32_t device;          /* HIP device ordinal */
 } agimp_config;
 
@@ -88,8 +92,12 @@ int agimp_substep(agimp_handle* h, float dt);
  * (used when a predicted octree is reused; not in the reference). */
 int agimp_substep_fixed_topology(agimp_handle* h, float dt);
 
-/* n substeps back to back on the device, no host round trips. */
+/* n substeps back to back on the device, no host round trips: one substep is captured in a
+ * hipGraph on first use and replayed n times (eager launches while timing is on). */
 int agimp_advance(agimp_handle* h, float dt, int32_t n);
+
+/* Particles per P2G work chunk (the autotuned parameter), AGIMP_MIN_CHUNK..AGIMP_MAX_CHUNK. */
+int agimp_set_particles_per_chunk(agimp_handle* h, int32_t ch);
 
 /* Read particle fields (adaptive_gimp.py:55-62) in original order; any pointer may be NULL. */
 int agimp_get_particles(agimp_handle* h, float* x, float* v, float* F, float* C, float* Jp, int32_t* fl,
@@ -125,6 +133,17 @@ int agimp_level_extent(agimp_handle* h, int32_t l);
 
 int agimp_sync(agimp_handle* h);
 const char* agimp_last_error(agimp_handle* h);
+
+/* Batches (not in the reference): many independent simulators of one dimension advanced
+ * together.  Every kernel of a substep is launched once for all scenes (one workgroup row
+ * per scene), and the whole substep is captured in one hipGraph and replayed.  The handles
+ * keep ownership of their state and stay usable on their own; do not use them concurrently
+ * with the batch.  dt[i] is scene i's time step; agimp_batch_advance returns when done. */
+int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out);
+int agimp_batch_destroy(agimp_batch* b);
+int agimp_batch_set_particles_per_chunk(agimp_batch* b, const int32_t* ch);
+int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps);
+const char* agimp_batch_last_error(agimp_batch* b);
 
 /* Host-side FP64 maths used by the kernels, exported for CPU checks: the rotation-variant
  * SVD (dim 2 or 3, row-major) and the constitutive update (clamp plasticity if enabled in
