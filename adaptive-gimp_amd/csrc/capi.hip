@@ -17,8 +17,8 @@
 using namespace agimp;
 
 // One captured substep (refine + bucketing + physics) of a set of scenes, replayed with
-// hipGraphLaunch.  Kernels read dt and the particle-buffer parity from device memory, so
-// the graph stays valid when dt changes.
+// hipGraphLaunch.  Kernels read dt from the device descriptors, so a graph stays valid when
+// dt changes; one graph is captured per value of the current-buffer index (G2P swaps them).
 struct Captured {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -38,12 +38,12 @@ struct agimp_handle {
   SceneDesc hd{};             // host copy of the scene descriptor
   SceneDesc* dd = nullptr;    // device copy (one entry)
   Dims dims{};
-  int host_parity = 0;        // mirrors *hd.parity
+  int host_parity = 0;        // index of the current particle buffer in hd.P
   int has_particles = 0;
   int topo_valid = 0;
   int max_chunks = 0;
   float graph_dt = -1.f;      // dt stored in the device descriptor for graph replays
-  Captured graph;
+  Captured graph[2];          // one substep starting from buffer 0 / buffer 1
   std::vector<void*> allocs;
   std::string err;
   // phase timing with HIP events on the handle's stream (agimp_set_timing)
@@ -57,11 +57,13 @@ struct agimp_handle {
 
 struct agimp_batch {
   std::vector<agimp_handle*> scenes;
+  std::vector<SceneDesc> host;  // batch copies of the descriptors, buffers ordered so that
+  int par = 0;                  // host[i].P[par] is scene i's current buffer
   int device = 0;
   hipStream_t stream = nullptr;
   SceneDesc* dd = nullptr;
   Dims dims{};
-  Captured graph;
+  Captured graph[2];
   std::string err;
 };
 
@@ -191,7 +193,7 @@ Dims single_dims(const agimp_handle* h) {
 
 int do_refine(agimp_handle* h) {
   PhaseScope ps(h, AGIMP_PHASE_REFINE);
-  launch_refine(h->stream, h->dd, h->dims);
+  launch_refine(h->stream, h->dd, h->dims, h->host_parity);
   h->topo_valid = 1;
   HIPCHK(h, hipGetLastError());
   return 0;
@@ -200,11 +202,11 @@ int do_refine(agimp_handle* h) {
 int do_physics(agimp_handle* h, float dt) {
   {
     PhaseScope ps(h, AGIMP_PHASE_BUCKET);
-    launch_bucket(h->stream, h->dd, h->dims);
+    launch_bucket(h->stream, h->dd, h->dims, h->host_parity);
   }
   {
     PhaseScope ps(h, AGIMP_PHASE_P2G);
-    launch_p2g(h->stream, h->dd, h->dims, dt);
+    launch_p2g(h->stream, h->dd, h->dims, dt, h->host_parity);
   }
   {
     PhaseScope ps(h, AGIMP_PHASE_GRID);
@@ -212,7 +214,7 @@ int do_physics(agimp_handle* h, float dt) {
   }
   {
     PhaseScope ps(h, AGIMP_PHASE_G2P);
-    launch_g2p(h->stream, h->dd, h->dims, dt);
+    launch_g2p(h->stream, h->dd, h->dims, dt, h->host_parity);
   }
   h->host_parity ^= 1;
   HIPCHK(h, hipGetLastError());
@@ -224,15 +226,15 @@ int do_physics(agimp_handle* h, float dt) {
 }
 
 // capture one full substep of the scenes behind `dd` into `cap` (dt taken from the descriptors)
-hipError_t capture_substep(hipStream_t s, const SceneDesc* dd, const Dims& m, Captured& cap) {
+hipError_t capture_substep(hipStream_t s, const SceneDesc* dd, const Dims& m, int par, Captured& cap) {
   cap.release();
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  launch_refine(s, dd, m);
-  launch_bucket(s, dd, m);
-  launch_p2g(s, dd, m, 0.f);
+  launch_refine(s, dd, m, par);
+  launch_bucket(s, dd, m, par);
+  launch_p2g(s, dd, m, 0.f, par);
   launch_grid(s, dd, m, 0.f);
-  launch_g2p(s, dd, m, 0.f);
+  launch_g2p(s, dd, m, 0.f, par);
   hipError_t le = hipGetLastError();
   e = hipStreamEndCapture(s, &cap.graph);
   if (le != hipSuccess) return le;
@@ -368,7 +370,6 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
                                          (int64_t)1 << 30);
   rc = rc ? rc : dalloc(h, &S.chunks, h->max_chunks);
   rc = rc ? rc : dalloc(h, &sd.cnt, NCOUNTERS);
-  rc = rc ? rc : dalloc(h, &sd.parity, 1);
   rc = rc ? rc : dalloc(h, &h->dd, 1);
   if (rc) { agimp_destroy(h); return rc; }
   sd.ph = make_physics(c);
@@ -385,7 +386,8 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
 int agimp_destroy(agimp_handle* h) {
   if (!h) return AGIMP_EINVAL;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
-  h->graph.release();
+  h->graph[0].release();
+  h->graph[1].release();
   for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
   for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
@@ -422,8 +424,8 @@ int agimp_set_particles(agimp_handle* h, const float* x, const float* v, const f
   e = e ? e : hipMemcpyAsync(dg, g, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
   if (Jp && e == hipSuccess) e = hipMemcpyAsync(dJ, Jp, (size_t)n * 4, hipMemcpyHostToDevice, h->stream);
   if (e == hipSuccess) {
-    launch_upload(h->stream, h->dd, D, n, dx, dv, dF, dC, dm, dg, Jp ? dJ : nullptr);
-    launch_stress(h->stream, h->dd, D, n);
+    launch_upload(h->stream, h->dd, h->host_parity, D, n, dx, dv, dF, dC, dm, dg, Jp ? dJ : nullptr);
+    launch_stress(h->stream, h->dd, D, n, h->host_parity);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -435,7 +437,8 @@ int agimp_set_particles(agimp_handle* h, const float* x, const float* v, const f
 
 int agimp_set_static_levels(agimp_handle* h, const int8_t* ret_level, const int8_t* act_level) {
   if (!h) return AGIMP_EINVAL;
-  h->graph.release();
+  h->graph[0].release();
+  h->graph[1].release();
   if (!ret_level || !act_level) {
     h->hd.static_mode = 0;
     h->dims = single_dims(h);
@@ -507,15 +510,18 @@ int agimp_advance(agimp_handle* h, float dt, int32_t n) {
     if (rc) return rc;
     h->graph_dt = dt;
   }
-  if (!h->graph.exec) {
-    hipError_t e = capture_substep(h->stream, h->dd, h->dims, h->graph);
+  for (int p = 0; p < 2; ++p) {
+    if (h->graph[p].exec) continue;
+    hipError_t e = capture_substep(h->stream, h->dd, h->dims, p, h->graph[p]);
     if (e != hipSuccess) {
-      h->graph.release();
+      h->graph[p].release();
       return fail(h, AGIMP_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
     }
   }
-  for (int i = 0; i < n; ++i) HIPCHK(h, hipGraphLaunch(h->graph.exec, h->stream));
-  h->host_parity ^= (n & 1);
+  for (int i = 0; i < n; ++i) {
+    HIPCHK(h, hipGraphLaunch(h->graph[h->host_parity].exec, h->stream));
+    h->host_parity ^= 1;
+  }
   h->topo_valid = 1;
   return 0;
 }
@@ -535,7 +541,7 @@ int agimp_get_particles(agimp_handle* h, float* x, float* v, float* F, float* C,
   float* dJ = dC + nm;
   int* dfl = (int*)(dJ + n);
   int* dg = dfl + n;
-  launch_download(h->stream, h->dd, D, n, dx, dv, dF, dC, dJ, dfl, dg);
+  launch_download(h->stream, h->dd, h->host_parity, D, n, dx, dv, dF, dC, dJ, dfl, dg);
   hipError_t e = hipGetLastError();
   if (x && e == hipSuccess) e = hipMemcpyAsync(x, dx, nv * 4, hipMemcpyDeviceToHost, h->stream);
   if (v && e == hipSuccess) e = hipMemcpyAsync(v, dv, nv * 4, hipMemcpyDeviceToHost, h->stream);
@@ -670,9 +676,9 @@ int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out
     agimp_batch_destroy(b);
     return AGIMP_EHIP;
   }
-  std::vector<SceneDesc> host(n);
-  for (int i = 0; i < n; ++i) host[i] = b->scenes[i]->hd;
-  if (hipMemcpy(b->dd, host.data(), sizeof(SceneDesc) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
+  b->host.resize(n);
+  for (int i = 0; i < n; ++i) b->host[i] = b->scenes[i]->hd;
+  if (hipMemcpy(b->dd, b->host.data(), sizeof(SceneDesc) * (size_t)n, hipMemcpyHostToDevice) != hipSuccess) {
     agimp_batch_destroy(b);
     return AGIMP_EHIP;
   }
@@ -680,10 +686,26 @@ int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out
   return 0;
 }
 
+// make every scene's batch descriptor list its current buffer at index b->par (the scene may
+// have been advanced on its own since the last batch call); uploads entries that changed
+static int batch_sync_buffers(agimp_batch* b) {
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    agimp_handle* h = b->scenes[i];
+    SceneDesc want = h->hd;
+    want.P[b->par] = h->hd.P[h->host_parity];
+    want.P[b->par ^ 1] = h->hd.P[h->host_parity ^ 1];
+    if (std::memcmp(&want, &b->host[i], sizeof(SceneDesc)) == 0) continue;
+    b->host[i] = want;
+    if (hipMemcpy(b->dd + i, &want, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess) return AGIMP_EHIP;
+  }
+  return 0;
+}
+
 int agimp_batch_destroy(agimp_batch* b) {
   if (!b) return AGIMP_EINVAL;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  b->graph.release();
+  b->graph[0].release();
+  b->graph[1].release();
   if (b->dd) (void)hipFree(b->dd);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -697,10 +719,8 @@ int agimp_batch_set_particles_per_chunk(agimp_batch* b, const int32_t* ch) {
     b->scenes[i]->hd.CH = ch[i];
     int rc = upload_desc(b->scenes[i]);
     if (rc) return rc;
-    if (hipMemcpy(b->dd + i, &b->scenes[i]->hd, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess)
-      return AGIMP_EHIP;
   }
-  return 0;
+  return batch_sync_buffers(b);
 }
 
 int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps) {
@@ -712,20 +732,23 @@ int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps) {
     if (h->hd.ph.dt != dt[i]) {
       h->hd.ph.dt = dt[i];
       h->graph_dt = -1.f;
-      if (hipMemcpy(b->dd + i, &h->hd, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess) return AGIMP_EHIP;
       if (hipMemcpy(h->dd, &h->hd, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess) return AGIMP_EHIP;
     }
   }
-  if (!b->graph.exec) {
-    hipError_t e = capture_substep(b->stream, b->dd, b->dims, b->graph);
+  if (batch_sync_buffers(b)) return AGIMP_EHIP;
+  for (int p = 0; p < 2; ++p) {
+    if (b->graph[p].exec) continue;
+    hipError_t e = capture_substep(b->stream, b->dd, b->dims, p, b->graph[p]);
     if (e != hipSuccess) {
-      b->graph.release();
+      b->graph[p].release();
       b->err = std::string("graph capture: ") + hipGetErrorString(e);
       return AGIMP_EHIP;
     }
   }
-  for (int s = 0; s < steps; ++s)
-    if (hipGraphLaunch(b->graph.exec, b->stream) != hipSuccess) return AGIMP_EHIP;
+  for (int s = 0; s < steps; ++s) {
+    if (hipGraphLaunch(b->graph[b->par].exec, b->stream) != hipSuccess) return AGIMP_EHIP;
+    b->par ^= 1;
+  }
   if (hipStreamSynchronize(b->stream) != hipSuccess) return AGIMP_EHIP;
   for (agimp_handle* h : b->scenes) {
     h->host_parity ^= (steps & 1);

@@ -101,8 +101,8 @@ struct SortBufs {
 
 // Everything the kernels need about one scene.  Kernels take an array of these and pick
 // theirs with blockIdx.y, so one launch advances a whole batch of scenes; a single
-// simulator is a batch of one.  The current particle buffer is P[*parity] (device-side, so
-// that a captured launch sequence can be replayed).
+// simulator is a batch of one.  Which of the two particle buffers is current is a launch
+// argument (the host tracks it; a captured substep exists for each of the two values).
 struct SceneDesc {
   Topo T;
   PBuf P[2];
@@ -115,7 +115,6 @@ struct SceneDesc {
   const int8_t* st_act;
   unsigned long long* cnt;
   uint32_t* vflag;
-  int* parity;
   size_t w_cells, w_node, w_flags;  // 32-bit words of the cell masks, node mask, block flags
 };
 

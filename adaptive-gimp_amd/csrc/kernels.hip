@@ -31,12 +31,11 @@ namespace agimp {
 template <int D>
 struct Pow4 { static constexpr int v = (D == 2) ? 16 : 64; };
 
-// the scene of this workgroup (blockIdx.y) and its current particle buffer
+// the scene of this workgroup (blockIdx.y); kernels that touch particles also take the
+// index `par` of the current particle buffer (the other one receives G2P's output)
 #define AGIMP_SCENE                          \
   const SceneDesc& sd = Dsc[blockIdx.y];     \
-  const Topo& T = sd.T;                      \
-  const int par = *sd.parity;                \
-  (void)par;
+  const Topo& T = sd.T;
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -122,46 +121,33 @@ __device__ __forceinline__ void mark_valid_node(const Topo& T, const int* f) {
   }
 }
 
-// zero the per-substep buffers of every scene in one launch
-struct FillRanges {
-  uint32_t* p[8];
-  size_t n[8];
-  int k;
-};
-
-__device__ __forceinline__ void fill_zero(const FillRanges& R) {
-  size_t total = 0;
-  for (int r = 0; r < R.k; ++r) total += R.n[r];
+// zero the per-substep buffers of every scene in one launch (one grid-stride loop per buffer)
+__device__ __forceinline__ void fill_zero(uint32_t* p, size_t n) {
   size_t st = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += st) {
-    size_t j = i;
-    int r = 0;
-    while (j >= R.n[r]) { j -= R.n[r]; ++r; }
-    R.p[r][j] = 0u;
-  }
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += st) p[i] = 0u;
 }
 
 // before refine: cell masks, node mask, block flags, list counters
-__global__ void k_clear_refine(const SceneDesc* Dsc) {
+__global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
-  FillRanges R{{(uint32_t*)T.lv[0].cell, (uint32_t*)T.node, (uint32_t*)T.lv[0].sflag, (uint32_t*)T.cflag,
-                (uint32_t*)T.scount, (uint32_t*)T.ccount},
-               {sd.w_cells, sd.w_node, sd.w_flags, sd.w_flags, (size_t)MAXL, 4},
-               6};
-  fill_zero(R);
+  fill_zero((uint32_t*)T.lv[0].cell, sd.w_cells);
+  fill_zero((uint32_t*)T.node, sd.w_node);
+  fill_zero((uint32_t*)T.lv[0].sflag, sd.w_flags);
+  fill_zero((uint32_t*)T.cflag, sd.w_flags);
+  fill_zero((uint32_t*)T.scount, MAXL);
+  fill_zero((uint32_t*)T.ccount, 4);
 }
 
 // before bucketing: block counters, the frozen counter, grid-phase list counters
-__global__ void k_clear_bucket(const SceneDesc* Dsc) {
+__global__ void k_clear_bucket(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
-  FillRanges R{{(uint32_t*)sd.S.bcnt, (uint32_t*)sd.S.frozen, (uint32_t*)T.scount},
-               {(size_t)T.total_blocks, 1, (size_t)MAXL},
-               3};
-  fill_zero(R);
+  fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
+  fill_zero((uint32_t*)sd.S.frozen, 1);
+  fill_zero((uint32_t*)T.scount, MAXL);
 }
 
 // the valid-node block flags of the octree: save after refine (to_saved) or restore
-__global__ void k_copy_flags(const SceneDesc* Dsc, int to_saved) {
+__global__ void k_copy_flags(const SceneDesc* __restrict__ Dsc, int to_saved) {
   AGIMP_SCENE
   uint32_t* a = (uint32_t*)T.lv[0].sflag;
   uint32_t* v = sd.vflag;
@@ -172,13 +158,8 @@ __global__ void k_copy_flags(const SceneDesc* Dsc, int to_saved) {
   }
 }
 
-// flip the current particle buffer (end of a physics step)
-__global__ void k_flip(const SceneDesc* Dsc) {
-  if (threadIdx.x == 0) *Dsc[blockIdx.y].parity ^= 1;
-}
-
 // append the ids of set flags to list (order unspecified)
-__global__ void k_compact_cells(const SceneDesc* Dsc, int which) {
+__global__ void k_compact_cells(const SceneDesc* __restrict__ Dsc, int which) {
   AGIMP_SCENE
   const uint8_t* flags = T.cflag;
   const int n = T.total_blocks;
@@ -192,7 +173,7 @@ __global__ void k_compact_cells(const SceneDesc* Dsc, int which) {
 
 // ------------------------------------------------------------------------------ clear
 template <int D>
-__global__ void k_clear_sgrid(const SceneDesc* Dsc) {
+__global__ void k_clear_sgrid(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   size_t total = 0;
   for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
@@ -215,7 +196,7 @@ __global__ void k_clear_sgrid(const SceneDesc* Dsc) {
 // ------------------------------------------------------------------------------ activation
 // adaptive_gimp.py:87-96: OR ACTIVATED into cell_mask[g_p][base + dI], dI in {0,1}^d
 template <int D>
-__global__ void k_activate_dynamic(const SceneDesc* Dsc) {
+__global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
   AGIMP_SCENE
   if (sd.static_mode) return;
   const PBuf& P = sd.P[par];
@@ -270,7 +251,7 @@ __device__ __forceinline__ int static_gi(const Topo& T, const int8_t* ret, const
 }
 
 template <int D>
-__global__ void k_activate_static(const SceneDesc* Dsc) {
+__global__ void k_activate_static(const SceneDesc* __restrict__ Dsc, int par) {
   AGIMP_SCENE
   if (!sd.static_mode) return;
   const PBuf& P = sd.P[par];
@@ -312,7 +293,7 @@ __global__ void k_activate_static(const SceneDesc* Dsc) {
 // ------------------------------------------------------------------------------ culling
 // adaptive_gimp.py:98-109 for all l >= 1 at once (ACTIVATED bits do not change here)
 template <int D>
-__global__ void k_mark_hier(const SceneDesc* Dsc) {
+__global__ void k_mark_hier(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   if (sd.static_mode) return;
   const int* list = T.clist;
@@ -350,7 +331,7 @@ __global__ void k_mark_hier(const SceneDesc* Dsc) {
 // adaptive_gimp.py:111-118 for all l at once: children of CULLING cells get ACTIVATED.
 // The culled cell itself keeps A|C (it compares unequal to ACTIVATED from here on).
 template <int D>
-__global__ void k_split(const SceneDesc* Dsc) {
+__global__ void k_split(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   if (sd.static_mode) return;
   const int* list = T.clist;
@@ -378,7 +359,7 @@ __global__ void k_split(const SceneDesc* Dsc) {
 // ------------------------------------------------------------------------------ node marks
 // adaptive_gimp.py:142-153: corners of exactly-ACTIVATED cells get ACTIVATED
 template <int D>
-__global__ void k_node_A(const SceneDesc* Dsc) {
+__global__ void k_node_A(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* list = T.clist;
   const int* countp = T.ccount + 2;
@@ -428,7 +409,7 @@ __device__ __forceinline__ int face_parent(const Topo& T, int l, const int* I, i
 
 // adaptive_gimp.py:170-174 for all levels at once
 template <int D>
-__global__ void k_mark_T(const SceneDesc* Dsc) {
+__global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* list = T.clist;
   const int* countp = T.ccount + 2;
@@ -471,7 +452,7 @@ __global__ void k_mark_T(const SceneDesc* Dsc) {
 
 // adaptive_gimp.py:176-183 for all levels at once, after every T mark
 template <int D>
-__global__ void k_mark_G(const SceneDesc* Dsc) {
+__global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* list = T.clist;
   const int* countp = T.ccount + 2;
@@ -514,7 +495,7 @@ __global__ void k_mark_G(const SceneDesc* Dsc) {
 // adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
 // particle's 2^d cells; rank inside its (level, leaf block) bucket by run-aggregated atomics.
 template <int D>
-__global__ void k_levelmap(const SceneDesc* Dsc) {
+__global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
   const int n = sd.n;
@@ -601,7 +582,7 @@ __device__ __forceinline__ int2 load_cc(const int* bcnt, int i, int n, int CH) {
   return make_int2(c, (c + CH - 1) / CH);
 }
 
-__global__ void k_scan_reduce(const SceneDesc* Dsc) {
+__global__ void k_scan_reduce(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* bcnt = sd.S.bcnt;
   const int n = T.total_blocks, CH = sd.CH;
@@ -617,7 +598,7 @@ __global__ void k_scan_reduce(const SceneDesc* Dsc) {
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_tiles(const SceneDesc* Dsc) {
+__global__ void k_scan_tiles(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   int2* tile_sum = sd.S.tiles;
   const int ntiles = (T.total_blocks + SCAN_TILE - 1) / SCAN_TILE;
@@ -635,7 +616,7 @@ __global__ void k_scan_tiles(const SceneDesc* Dsc) {
   if (threadIdx.x == 0) { totals[0] = carry.x; totals[1] = carry.y; }
 }
 
-__global__ void k_scan_final(const SceneDesc* Dsc) {
+__global__ void k_scan_final(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* bcnt = sd.S.bcnt;
   const int n = T.total_blocks, CH = sd.CH;
@@ -658,7 +639,7 @@ __global__ void k_scan_final(const SceneDesc* Dsc) {
   }
 }
 
-__global__ void k_scatter(const SceneDesc* Dsc) {
+__global__ void k_scatter(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int n = sd.n;
   const int* gb = sd.S.gb;
@@ -681,7 +662,7 @@ __global__ void k_scatter(const SceneDesc* Dsc) {
 
 // per occupied leaf block: P2G chunk descriptors + the blocks its 3^d stencils touch
 template <int D>
-__global__ void k_blocks(const SceneDesc* Dsc) {
+__global__ void k_blocks(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* bcnt = sd.S.bcnt;
   const int* boff = sd.S.boff;
@@ -714,7 +695,7 @@ __global__ void k_blocks(const SceneDesc* Dsc) {
 
 // compact the grid-phase block flags of every level into per-level lists
 template <int D>
-__global__ void k_scompact(const SceneDesc* Dsc) {
+__global__ void k_scompact(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   if (blockIdx.x * blockDim.x >= (unsigned)T.total_blocks) return;  // whole workgroup past this scene
   int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -796,7 +777,7 @@ constexpr int P2G_MAX_CH = 2048; // largest work chunk (particles of one leaf bl
 //     tile (no two lanes of one instruction hit the same LDS word);
 //  3. the tile is flushed with one float atomic per non-zero node component.
 template <int D>
-__global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* Dsc, float dt_arg) {
+__global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
   int* perm = sd.S.perm;
@@ -1022,7 +1003,7 @@ __device__ __forceinline__ bool in_range_nodes(const int* I, int n) {
 // adaptive_gimp.py:289-301 as a gather onto each valid coarse node I of level l-1:
 // sources are fine nodes 2I - d with weight 2^-|d|_1.
 template <int D>
-__global__ void k_restrict(const SceneDesc* Dsc, int l) {
+__global__ void k_restrict(const SceneDesc* __restrict__ Dsc, int l) {
   AGIMP_SCENE
   if (l >= T.L) return;
   const LevelDesc& Lc = T.lv[l - 1];
@@ -1060,7 +1041,7 @@ __global__ void k_restrict(const SceneDesc* Dsc, int l) {
 
 // adaptive_gimp.py:277-287 on every level's list (+ sticky walls / slab collider)
 template <int D>
-__global__ void k_gridop(const SceneDesc* Dsc, float dt_arg) {
+__global__ void k_gridop(const SceneDesc* __restrict__ Dsc, float dt_arg) {
   AGIMP_SCENE
   Physics ph = sd.ph;
   if (dt_arg > 0.f) ph.dt = dt_arg;
@@ -1112,7 +1093,7 @@ __global__ void k_gridop(const SceneDesc* Dsc, float dt_arg) {
 // adaptive_gimp.py:303-321 as a gather onto each level-l node J that is T, ghost or
 // all-even: J = 2I + d over valid coarse nodes I, weight 2^-|d|_1 (only d = 0 unless T/G).
 template <int D>
-__global__ void k_prolong(const SceneDesc* Dsc, int l) {
+__global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l) {
   AGIMP_SCENE
   if (l >= T.L) return;
   const LevelDesc& Lf = T.lv[l];
@@ -1162,7 +1143,7 @@ __global__ void k_prolong(const SceneDesc* Dsc, int l) {
 // adaptive_gimp.py:252-275 (+ plasticity and the next substep's stress); reads slot
 // perm[pos] of the current buffer, writes slot pos of the next buffer (re-sorting for free).
 template <int D>
-__global__ void __launch_bounds__(256) k_g2p(const SceneDesc* Dsc, float dt_arg) {
+__global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
   AGIMP_SCENE
   const PBuf& Pin = sd.P[par];
   const PBuf& Pout = sd.P[par ^ 1];
@@ -1281,7 +1262,7 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* Dsc, float dt_arg)
 
 // Kirchhoff stress of the stored F (no plastic projection): after the caller sets state
 template <int D>
-__global__ void k_stress(const SceneDesc* Dsc) {
+__global__ void k_stress(const SceneDesc* __restrict__ Dsc, int par) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
   const int n = sd.n;
@@ -1305,7 +1286,7 @@ __global__ void k_stress(const SceneDesc* Dsc) {
 // ------------------------------------------------------------------------------ host I/O
 // caller order (AoS, like the reference's to_numpy layout) -> slot k = original index
 template <int D>
-__global__ void k_upload(const SceneDesc* Dsc, const float* x, const float* v, const float* F, const float* C,
+__global__ void k_upload(const SceneDesc* __restrict__ Dsc, int par, const float* x, const float* v, const float* F, const float* C,
                          const float* m, const int* g, const float* Jp) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
@@ -1333,7 +1314,7 @@ __global__ void k_upload(const SceneDesc* Dsc, const float* x, const float* v, c
 
 // slot order -> caller order (AoS)
 template <int D>
-__global__ void k_download(const SceneDesc* Dsc, float* x, float* v, float* F, float* C, float* Jp, int* fl,
+__global__ void k_download(const SceneDesc* __restrict__ Dsc, int par, float* x, float* v, float* F, float* C, float* Jp, int* fl,
                            int* g) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
@@ -1361,7 +1342,7 @@ __global__ void k_download(const SceneDesc* Dsc, float* x, float* v, float* F, f
 // dense readback of one level over [0, e)^d; culled cells (A|C kept for speed) read as the
 // reference's 0 (or GHOST when a ghost mark landed on them later)
 template <int D>
-__global__ void k_dense_level(const SceneDesc* Dsc, int l, int* cell, float* m, float* v) {
+__global__ void k_dense_level(const SceneDesc* __restrict__ Dsc, int l, int* cell, float* m, float* v) {
   AGIMP_SCENE
   const LevelDesc& Ll = T.lv[l];
   size_t tot = 1;
@@ -1383,7 +1364,7 @@ __global__ void k_dense_level(const SceneDesc* Dsc, int l, int* cell, float* m, 
 }
 
 template <int D>
-__global__ void k_dense_node(const SceneDesc* Dsc, int* out) {
+__global__ void k_dense_node(const SceneDesc* __restrict__ Dsc, int* out) {
   AGIMP_SCENE
   const LevelDesc& Ll = T.lv[T.L - 1];
   size_t tot = 1;
@@ -1413,7 +1394,7 @@ static inline dim3 grid2(size_t x, int nscenes) {
   return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(x, 1u << 20)), (unsigned)nscenes);
 }
 
-void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m) {
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 bg = grid2(blocks_for(m.max_total_blocks, 256), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
@@ -1421,9 +1402,9 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m) {
   AGIMP_DIM(m.dim, {
     hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_clear_refine, fg, dim3(256), 0, s, d);
-    if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d);
+    if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
     if (m.any_dynamic) {
-      hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d);
+      hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
       if (m.maxL > 1) {
         hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 0);
         hipLaunchKernelGGL(k_mark_hier<DD>, lg, dim3(256), 0, s, d);
@@ -1439,7 +1420,7 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m) {
   });
 }
 
-void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m) {
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 bg = grid2(blocks_for(m.max_total_blocks, 256), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
@@ -1449,7 +1430,7 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m) {
     hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_clear_bucket, fg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 0);
-    hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
     hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_tiles, grid2(1, m.nscenes), dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);
@@ -1459,8 +1440,8 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m) {
   });
 }
 
-void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
-  AGIMP_DIM(m.dim, hipLaunchKernelGGL(k_p2g<DD>, grid2(m.p2g_grid, m.nscenes), dim3(P2G_T), 0, s, d, dt));
+void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par) {
+  AGIMP_DIM(m.dim, hipLaunchKernelGGL(k_p2g<DD>, grid2(m.p2g_grid, m.nscenes), dim3(P2G_T), 0, s, d, dt, par));
 }
 
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
@@ -1472,25 +1453,25 @@ void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
   });
 }
 
-void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
-  AGIMP_DIM(m.dim, hipLaunchKernelGGL(k_g2p<DD>, grid2(blocks_for(m.max_n, 256), m.nscenes), dim3(256), 0, s, d, dt));
-  hipLaunchKernelGGL(k_flip, grid2(1, m.nscenes), dim3(64), 0, s, d);
+void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par) {
+  AGIMP_DIM(m.dim,
+            hipLaunchKernelGGL(k_g2p<DD>, grid2(blocks_for(m.max_n, 256), m.nscenes), dim3(256), 0, s, d, dt, par));
 }
 
-void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_stress<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d));
+void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n, int par) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_stress<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par));
 }
 
-void launch_upload(hipStream_t s, const SceneDesc* d, int dim, int n, const float* x, const float* v, const float* F,
-                   const float* C, const float* m, const int* g, const float* Jp) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_upload<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, x, v, F, C, m, g,
-                                    Jp));
+void launch_upload(hipStream_t s, const SceneDesc* d, int par, int dim, int n, const float* x, const float* v,
+                   const float* F, const float* C, const float* m, const int* g, const float* Jp) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_upload<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par, x, v, F, C, m,
+                                    g, Jp));
 }
 
-void launch_download(hipStream_t s, const SceneDesc* d, int dim, int n, float* x, float* v, float* F, float* C,
-                     float* Jp, int* fl, int* g) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_download<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, x, v, F, C, Jp,
-                                    fl, g));
+void launch_download(hipStream_t s, const SceneDesc* d, int par, int dim, int n, float* x, float* v, float* F,
+                     float* C, float* Jp, int* fl, int* g) {
+  AGIMP_DIM(dim, hipLaunchKernelGGL(k_download<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par, x, v, F, C,
+                                    Jp, fl, g));
 }
 
 void launch_dense_level(hipStream_t s, const SceneDesc* d, const Topo& T, int l, int* cell, float* m, float* v) {

@@ -21,16 +21,17 @@ struct Dims {
 };
 
 size_t cells_total(const Topo& T, int lmin, int lmax);
-void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m);
-void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m);
-void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt);
+// par: index of the current particle buffer of every scene of the launch (G2P writes par ^ 1)
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par);
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par);
+void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt);
-void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt);
-void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n);
-void launch_upload(hipStream_t s, const SceneDesc* d, int dim, int n, const float* x, const float* v, const float* F,
-                   const float* C, const float* m, const int* g, const float* Jp);
-void launch_download(hipStream_t s, const SceneDesc* d, int dim, int n, float* x, float* v, float* F, float* C,
-                     float* Jp, int* fl, int* g);
+void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
+void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n, int par);
+void launch_upload(hipStream_t s, const SceneDesc* d, int par, int dim, int n, const float* x, const float* v,
+                   const float* F, const float* C, const float* m, const int* g, const float* Jp);
+void launch_download(hipStream_t s, const SceneDesc* d, int par, int dim, int n, float* x, float* v, float* F,
+                     float* C, float* Jp, int* fl, int* g);
 void launch_dense_level(hipStream_t s, const SceneDesc* d, const Topo& T, int l, int* cell, float* m, float* v);
 void launch_dense_node(hipStream_t s, const SceneDesc* d, const Topo& T, int* out);
 

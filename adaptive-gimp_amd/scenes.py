@@ -234,7 +234,7 @@ def _box_tags(rel, cuts):
     return g
 
 
-def c1_two_squares(seed=0, scale=1.0):
+def c1_two_squares(seed=0, scale=1.0, rho=1.0):
     """2D dynamic adaptivity: L=3, coarsest 64 (finest dx 1/256), two squares of side 0.2 at
     (0.3,0.6) and (0.7,0.4), 4 particles per finest cell on a jittered 2x2 sub-lattice
     (~20,972 particles), E=1e4, nu=0.2, rho=1, gravity (0,-9.8), singular-value clamp
@@ -253,7 +253,7 @@ def c1_two_squares(seed=0, scale=1.0):
         gs.append(_box_tags((p - lo) / 0.2, [0.1, 0.03]))
     x = np.concatenate(xs)
     g = np.concatenate(gs)
-    rho, ppc = 1.0, 4.0
+    ppc = 4.0
     p_vol = dxf ** 2 / ppc
     st = _state(x, np.zeros_like(x), g, rho * p_vol, 2)
     prm = dict(E=1e4, nu=0.2, gravity=(0.0, -9.8), p_vol=p_vol, wall_mode=1, plastic=1, theta_c=2.5e-2,
@@ -261,7 +261,7 @@ def c1_two_squares(seed=0, scale=1.0):
     return Workload("c1_2d_dynamic_two_squares", 2, L, c, st, 1e-4, 2000, prm)
 
 
-def c2_static_band(seed=0, n=200_000):
+def c2_static_band(seed=0, n=200_000, rho=1.0):
     """2D static adaptivity: L=4, coarsest 64 (finest dx 1/512), refinement fixed around
     y=0.3 (finest within |y-0.3|<=0.05, level 2 within 0.1, level 1 within 0.2, coarsest
     elsewhere; hierarchical so cells never overlap); 200k particles uniformly in 5 seeded
@@ -277,13 +277,10 @@ def c2_static_band(seed=0, n=200_000):
     counts[-1] += n - counts.sum()
     x = np.concatenate([los[b] + rng.random((counts[b], 2)) * sides[b] for b in range(5)])
     v = np.concatenate([np.tile(vel[b], (counts[b], 1)) for b in range(5)])
-    dxf = 1.0 / (c * 2 ** (L - 1))
-    rho = 1.0
     p_vol = float(area.sum()) / n
     st = _state(x, v, np.zeros(n, np.int32), rho * p_vol, 2)
     prm = dict(E=1e4, nu=0.2, gravity=(0.0, -9.8), p_vol=p_vol, wall_mode=1, plastic=1, theta_c=2.5e-2,
                theta_s=7.5e-3, hardening=0.0)
-    del dxf
     return Workload("c2_2d_static_band", 2, L, c, st, 5e-5, 5000, prm, grid_initializer=band_initializer())
 
 
@@ -333,10 +330,86 @@ def small_3d(seed=0, n=20_000, L=3, c=16):
     return Workload("small_3d_sphere", 3, L, c, st, 2e-5, 100, prm)
 
 
+def _c4_scene(i, rho):
+    """Scene i of the batched 2D workload (seed = scene index)."""
+    r = np.random.default_rng(i)
+    n = int(round(math.exp(r.uniform(math.log(5e3), math.log(2e5)))))
+    L = int(r.integers(2, 5))
+    finest = int(r.choice([256, 512]))
+    c = finest >> (L - 1)
+    nb = int(r.integers(1, 4))
+    sides = r.uniform(0.05, 0.25, (nb, 2))
+    los = r.uniform(0.1, 0.9 - sides)
+    vel = r.uniform(-1, 1, (nb, 2))
+    area = np.prod(sides, axis=1)
+    counts = np.floor(n * area / area.sum()).astype(int)
+    counts[-1] += n - counts.sum()
+    cuts = {2: [0.1], 3: [0.1, 0.03], 4: [0.2, 0.1, 0.03]}[L]
+    xs, vs, gs = [], [], []
+    for b in range(nb):
+        rel = r.random((counts[b], 2))
+        xs.append(los[b] + rel * sides[b])
+        vs.append(np.tile(vel[b], (counts[b], 1)))
+        gs.append(_box_tags(rel, cuts))
+    x, v, g = np.concatenate(xs), np.concatenate(vs), np.concatenate(gs)
+    p_vol = float(area.sum()) / n
+    st = _state(x, v, g, rho * p_vol, 2)
+    prm = dict(E=1e4, nu=0.2, gravity=(0.0, -9.8), p_vol=p_vol, wall_mode=1, plastic=1, theta_c=2.5e-2,
+               theta_s=7.5e-3, hardening=0.0)
+    return Workload(f"c4_scene{i}", 2, L, c, st, 5e-5, 1000, prm)
+
+
+def c4_batch(n_scenes=64, rho=1.0):
+    """Batched 2D workload: n_scenes independent scenes, scene i seeded with i: particle count
+    log-uniform in [5k, 200k], levels U{2..4}, finest dx in {1/256, 1/512} (coarsest size =
+    finest / 2^(L-1)), 1-3 boxes with sides U[0.05,0.25] and one velocity U[-1,1]^2 each,
+    level tags by relative position in the box (init.py:17-29 rule cut to L levels), dynamic
+    adaptivity, material of the static-band workload (E=1e4, nu=0.2, clamp, sticky walls),
+    dt=5e-5, 1000 substeps.  rho=1 as written violates the P-wave CFL bound at that dt."""
+    return [_c4_scene(i, rho) for i in range(n_scenes)]
+
+
+def c5_three_spheres(seed=0, n=4_000_000):
+    """3D learned-adaptivity workload (plain physics part): L=4, coarsest 64 (finest 1/512);
+    three spheres r=0.1 on a seeded triangle around (0.5,0.5,0.5), each moving towards the
+    centroid with a speed U[1,3]; n particles (default 4,000,000) drawn without replacement
+    from jittered lattices of spacing dx/2; snow material as the 1M sphere; level tags by
+    radial shell r/R <= 0.7 -> 0, <= 0.85 -> 1, <= 0.95 -> 2, else 3; dt=2e-5, 2000 substeps."""
+    rng = np.random.default_rng(seed)
+    L, c = 4, 64
+    dxf = 1.0 / (c * 2 ** (L - 1))
+    R = 0.1
+    phi0 = rng.uniform(0, 2 * math.pi)
+    centre = np.array([0.5, 0.5, 0.5])
+    pts, vels, gs = [], [], []
+    for k in range(3):
+        phi = phi0 + 2 * math.pi * k / 3
+        ck = centre + 0.16 * np.array([math.cos(phi), 0.3 * rng.uniform(-1, 1), math.sin(phi)])
+        p = sphere_lattice(ck, R, dxf / 2, rng)
+        speed = rng.uniform(1, 3)
+        d = centre - ck
+        v = speed * d / np.linalg.norm(d)
+        rr = np.sqrt(np.sum((p - ck) ** 2, axis=1)) / R
+        gs.append(np.where(rr <= 0.7, 0, np.where(rr <= 0.85, 1, np.where(rr <= 0.95, 2, 3))).astype(np.int32))
+        pts.append(p)
+        vels.append(np.tile(v, (p.shape[0], 1)))
+    x, v, g = np.concatenate(pts), np.concatenate(vels), np.concatenate(gs)
+    if n < x.shape[0]:
+        keep = np.sort(rng.choice(x.shape[0], n, replace=False))
+        x, v, g = x[keep], v[keep], g[keep]
+    p_vol = 3 * (4.0 / 3.0) * math.pi * R ** 3 / x.shape[0]
+    st = _state(x, v, g, 400.0 * p_vol, 3)
+    prm = dict(E=1.4e5, nu=0.2, gravity=(0.0, -9.8, 0.0), p_vol=p_vol, wall_mode=1, plastic=1, theta_c=2.5e-2,
+               theta_s=7.5e-3, hardening=10.0)
+    return Workload("c5_3d_three_spheres_4M_4lv", 3, L, c, st, 2e-5, 2000, prm)
+
+
 WORKLOADS = {
     "c1": c1_two_squares,
     "c2": c2_static_band,
     "c3": c3_sphere_on_slab,
+    "c4": c4_batch,
+    "c5": c5_three_spheres,
 }
 
 

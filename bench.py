@@ -11,7 +11,7 @@ transfer kernel (P2G or G2P; algorithmic bytes / average kernel time measured wi
 events on the library's stream) and a CPU baseline (the oracle's C++ OpenMP restatement on
 a bounded sample of the same workload, rank 0, N=1).
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--workload c3|c5|c2|c4] [--tune] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -51,30 +51,102 @@ def algorithmic_bytes(dim, n_particles, n_nodes):
     return p2g, g2p
 
 
-def cpu_baseline(wl, seconds_target=12.0):
-    """The oracle (float32, OpenMP) timed on this host: 1 warm-up substep, then substeps until
-    ~seconds_target have elapsed (at least 2).  Test infrastructure used only as the baseline."""
+def _oracle_for(wl, threads):
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
     ora = O.OracleSim(wl.dim, wl.level, wl.coarsest_size, wl.n, fp64=False, threads=threads, **wl.params)
     st = wl.state
     ora.set_particles(st["x"], st["v"], st["F"], st["C"], st["m"], st["g"])
-    ora.substep(wl.dt)
-    times = []
-    t_end = time.perf_counter() + seconds_target
-    while len(times) < 2 or time.perf_counter() < t_end:
-        t0 = time.perf_counter()
+    if wl.grid_initializer is not None:
+        import __graft_entry__ as ge
+
+        ret, act = ge.package().scenes.static_level_maps(wl.grid_initializer, wl.dim, wl.level, wl.coarsest_size)
+        ora.set_static_map(ret, act)
+    return ora
+
+
+def cpu_baseline(workloads, seconds_target=12.0):
+    """The oracle (float32 + FP64 SVD, C++ OpenMP) timed on this host's cores: per scene one
+    warm-up substep, then substeps until ~seconds_target have elapsed over all scenes (at
+    least 2 per scene).  Test infrastructure, used here only as the reported baseline."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    per_scene = seconds_target / len(workloads)
+    work, elapsed, nsub = 0.0, 0.0, 0
+    for wl in workloads:
+        ora = _oracle_for(wl, threads)
         ora.substep(wl.dt)
-        times.append(time.perf_counter() - t0)
-        if len(times) >= 50:
-            break
-    times.sort()
-    med = times[len(times) // 2]
-    return {"value": wl.n / med, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} substeps of the same {wl.n}-particle scene after 1 warm-up substep, "
-                      f"median {med * 1e3:.1f} ms/substep (oracle C++ OpenMP restatement, float32 + FP64 SVD)"}
+        t_end = time.perf_counter() + per_scene
+        k = 0
+        t0 = time.perf_counter()
+        while k < 2 or time.perf_counter() < t_end:
+            ora.substep(wl.dt)
+            k += 1
+            if k >= 50:
+                break
+        elapsed += time.perf_counter() - t0
+        work += wl.n * k
+        nsub += k
+        del ora
+    names = workloads[0].name if len(workloads) == 1 else f"{len(workloads)} scenes ({workloads[0].name}...)"
+    return {"value": work / elapsed, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"{nsub} substeps of {names} after one warm-up substep each "
+                      f"({work / 1e6:.1f} M particle-substeps in {elapsed:.1f} s; oracle C++ OpenMP "
+                      "restatement, float32 + FP64 SVD)"}
+
+
+METRIC = "particle-substeps/s (3D 1M p, 3 levels) plain vs learned-adapt; HBM GB/s"
+
+
+def bench_single(pkg, wl, args, np):
+    sim = pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(), wl.grid_initializer,
+                           particles_per_chunk=args.ppc, **wl.params)
+    tuned = None
+    if args.tune:
+        from importlib import import_module
+
+        at = import_module(pkg.__name__ + ".autotune")
+        tuned = at.tune(sim, wl.dt)
+    sim.advance(wl.dt, args.warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.advance(wl.dt, args.steps)  # one captured substep replayed: the production path
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    # per-phase breakdown: eager substeps with HIP events on the library's stream
+    sim.timings()
+    sim.set_timing(True)
+    for _ in range(min(args.steps, 10)):
+        sim.substep(wl.dt)
+    tim = sim.timings()
+    sim.set_timing(False)
+    n_nodes = int(sum(int(np.count_nonzero(sim.ad_grid_m[l].to_numpy())) for l in range(wl.level)))
+    counters = sim.counters()
+    sim.close()
+    return elapsed, tim, n_nodes, counters, tuned
+
+
+def bench_batch(pkg, wls, args):
+    sims = [pkg.AdaptiveGIMP(w.dim, w.level, w.coarsest_size, w.n, w.initializer(), w.grid_initializer,
+                             particles_per_chunk=256, **w.params) for w in wls]
+    batch = pkg.Batch(sims)
+    dts = [w.dt for w in wls]
+    tuned = None
+    if args.tune:
+        from importlib import import_module
+
+        at = import_module(pkg.__name__ + ".autotune")
+        ch, log = at.tune_batch(batch, dts)
+        tuned = {"chunks_by_class": sorted(set(int(c) for c in ch)), "trials": log}
+    batch.advance(dts, args.warmup)
+    t0 = time.perf_counter()
+    batch.advance(dts, args.steps)
+    elapsed = time.perf_counter() - t0
+    counters = [s.counters() for s in sims]
+    batch.close()
+    for s in sims:
+        s.close()
+    return elapsed, counters, tuned
 
 
 def main():
@@ -82,9 +154,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--workload", default="c3", choices=["c3", "c5", "c2", "c4"],
+                    help="c3: 3D 1M sphere (the headline); c5: 3D 4M three spheres; c2: 2D static band; "
+                         "c4: 64 batched 2D scenes")
+    ap.add_argument("--n", type=int, default=0, help="particle count override (c3/c5)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ppc", type=int, default=1024, help="P2G particles per chunk")
+    ap.add_argument("--tune", action="store_true", help="autotune the P2G chunk size first")
+    ap.add_argument("--ppc", type=int, default=1024, help="P2G particles per chunk (single-scene workloads)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run one plain device copy of this many MiB (a known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -109,63 +185,63 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    wl = pkg.scenes.c3_sphere_on_slab(n=args.n)
-    sim = pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(),
-                           particles_per_chunk=args.ppc, **wl.params)
-    for _ in range(args.warmup):
-        sim.substep(wl.dt)
-    sim.sync()
-    sim.timings()  # drop warm-up timings
-    sim.set_timing(True)
-    sim.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.substep(wl.dt)
-    sim.sync()
-    elapsed = time.perf_counter() - t0
-    tim = sim.timings()
-    sim.set_timing(False)
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = wl.n * args.steps / elapsed
-
-    # nodes carrying mass after the last substep, summed over levels (per-launch node count)
-    n_nodes = int(sum(int(np.count_nonzero(sim.ad_grid_m[l].to_numpy())) for l in range(wl.level)))
-    b_p2g, b_g2p = algorithmic_bytes(wl.dim, wl.n, n_nodes)
-    avg = {k: (v[0] / v[1] if v[1] else float("nan")) for k, v in tim.items()}
-    kern = {"p2g": (b_p2g, avg["p2g"]), "g2p": (b_g2p, avg["g2p"])}
-    dom = max(kern, key=lambda k: kern[k][1])
-    achieved = kern[dom][0] / (kern[dom][1] * 1e-3) / 1e9
-    counters = sim.counters()
-    phase_ms = {k: round(v, 4) for k, v in avg.items()}
-
-    line = {
-        "metric": "particle-substeps/s (3D 1M p, 3 levels) plain vs learned-adapt; HBM GB/s",
-        "value": value,
-        "unit": "particle-substeps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 (FP64 SVD/plasticity)",
-        "data": "synthetic (seeded jittered-lattice sphere)",
-        "config": {"workload": wl.name, "dim": wl.dim, "levels": wl.level, "coarsest_size": wl.coarsest_size,
-                   "n_particles": wl.n, "dt": wl.dt, "mode": "plain", "particles_per_chunk": args.ppc},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-                     "bytes_per_launch": kern[dom][0], "avg_kernel_ms": kern[dom][1], "active_nodes": n_nodes},
-        "phase_ms": phase_ms,
-        "p2g_gbs": b_p2g / (avg["p2g"] * 1e-3) / 1e9,
-        "g2p_gbs": b_g2p / (avg["g2p"] * 1e-3) / 1e9,
-        "counters": counters,
-    }
-    sim.close()
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(wl)
+    S = pkg.scenes
+    notes = None
+    if args.workload in ("c3", "c5"):
+        wl = S.c3_sphere_on_slab(n=args.n or 1_000_000) if args.workload == "c3" else \
+            S.c5_three_spheres(n=args.n or 4_000_000)
+        wls = [wl]
+    elif args.workload == "c2":
+        wls = [S.c2_static_band(rho=1000.0)]
+        notes = "rho=1000 instead of the stated 1 (rho=1 violates the P-wave CFL bound at dt=5e-5)"
     else:
-        line["cpu_baseline"] = None
+        wls = S.c4_batch(rho=1000.0)
+        notes = "rho=1000 instead of the stated 1 (rho=1 violates the P-wave CFL bound at dt=5e-5)"
+    total_n = sum(w.n for w in wls)
+    line = {"metric": METRIC, "unit": "particle-substeps/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32 (FP64 SVD/plasticity)", "data": "synthetic (seeded scene generators)"}
+    if len(wls) == 1:
+        wl = wls[0]
+        elapsed, tim, n_nodes, counters, tuned = bench_single(pkg, wl, args, np)
+        b_p2g, b_g2p = algorithmic_bytes(wl.dim, wl.n, n_nodes)
+        avg = {k: (v[0] / v[1] if v[1] else float("nan")) for k, v in tim.items()}
+        kern = {"p2g": (b_p2g, avg["p2g"]), "g2p": (b_g2p, avg["g2p"])}
+        dom = max(kern, key=lambda k: kern[k][1])
+        achieved = kern[dom][0] / (kern[dom][1] * 1e-3) / 1e9
+        line.update({
+            "value": total_n * args.steps / elapsed,
+            "ms_per_step": elapsed * 1e3 / args.steps,
+            "config": {"workload": wl.name, "dim": wl.dim, "levels": wl.level, "coarsest_size": wl.coarsest_size,
+                       "n_particles": wl.n, "dt": wl.dt, "mode": "plain", "timed_path": "hipGraph replay",
+                       "particles_per_chunk": tuned[0] if tuned else args.ppc},
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(dom) if wl.name.startswith("c3") else None,
+                         "bytes_per_launch": kern[dom][0], "avg_kernel_ms": kern[dom][1], "active_nodes": n_nodes},
+            "phase_ms_eager": {k: round(v, 4) for k, v in avg.items()},
+            "p2g_gbs": b_p2g / (avg["p2g"] * 1e-3) / 1e9,
+            "g2p_gbs": b_g2p / (avg["g2p"] * 1e-3) / 1e9,
+            "counters": counters,
+        })
+        if tuned:
+            line["autotune_s_per_substep"] = {str(k): v for k, v in tuned[1].items()}
+    else:
+        elapsed, counters, tuned = bench_batch(pkg, wls, args)
+        line.update({
+            "value": total_n * args.steps / elapsed,
+            "ms_per_step": elapsed * 1e3 / args.steps,
+            "config": {"workload": "c4_batched_2d_64_scenes", "scenes": len(wls), "n_particles_total": total_n,
+                       "levels": sorted(set(w.level for w in wls)), "mode": "plain",
+                       "timed_path": "one hipGraph replay per batched substep"},
+            "roofline": None,
+            "counters_total": {k: sum(c[k] for c in counters) for k in counters[0]},
+        })
+        if tuned:
+            line["autotune"] = tuned
+    if notes:
+        line["config"]["note"] = notes
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(wls if len(wls) <= 8 else wls[:8])
     print(json.dumps(line), flush=True)
 
 
