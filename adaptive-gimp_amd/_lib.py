@@ -41,7 +41,12 @@ EXPORTS = [
     "agimp_set_particles_per_chunk", "agimp_batch_create", "agimp_batch_destroy",
     "agimp_batch_set_particles_per_chunk", "agimp_batch_advance", "agimp_batch_last_error",
     "agimp_host_svd", "agimp_host_constitutive",
+    "agimp_learned_enable", "agimp_learned_set_model", "agimp_learned_features", "agimp_learned_labels_reset",
+    "agimp_learned_labels_accumulate", "agimp_learned_labels", "agimp_learned_predict", "agimp_learned_advance",
+    "agimp_learned_stats", "agimp_learned_reset_stats", "agimp_mlp_forward", "agimp_mlp_loss_grad", "agimp_mlp_adam",
 ]
+MLP_NPARAMS = 5508
+LSTATS = ["rebuilds", "forced", "discarded", "replayed", "substeps", "violations", "candidates", "refined"]
 MIN_CHUNK, MAX_CHUNK = 64, 2048
 
 _lib = None
@@ -65,6 +70,7 @@ def load():
     ip = ctypes.POINTER(ctypes.c_int32)
     i8p = ctypes.POINTER(ctypes.c_int8)
     dp = ctypes.POINTER(ctypes.c_double)
+    i64p = ctypes.POINTER(ctypes.c_int64)
     cfgp = ctypes.POINTER(AgimpConfig)
     sig = {
         "agimp_config_init": [cfgp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int64],
@@ -94,6 +100,20 @@ def load():
         "agimp_batch_last_error": [vp],
         "agimp_host_svd": [ctypes.c_int32, dp, dp, dp, dp],
         "agimp_host_constitutive": [cfgp, dp, dp, dp, ctypes.c_int32],
+        "agimp_learned_enable": [vp, ctypes.c_int32, ctypes.c_float],
+        "agimp_learned_set_model": [vp, fp, ctypes.c_int32],
+        "agimp_learned_features": [vp, ctypes.c_float, fp, ip, ctypes.c_int64, i64p],
+        "agimp_learned_labels_reset": [vp],
+        "agimp_learned_labels_accumulate": [vp],
+        "agimp_learned_labels": [vp, ip, ctypes.c_int64],
+        "agimp_learned_predict": [vp, ctypes.c_float, ip, i8p, ctypes.c_int64, i64p],
+        "agimp_learned_advance": [vp, ctypes.c_float, ctypes.c_int32],
+        "agimp_learned_stats": [vp, i64p, ctypes.c_int32],
+        "agimp_learned_reset_stats": [vp],
+        "agimp_mlp_forward": [vp, ctypes.c_int64, vp, vp, vp],
+        "agimp_mlp_loss_grad": [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_float, vp, vp, vp],
+        "agimp_mlp_adam": [vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                           ctypes.c_float, ctypes.c_float, vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 OUT = os.path.join(HERE, "libagimp.so")
-SOURCES = ["kernels.hip", "capi.hip"]
+SOURCES = ["kernels.hip", "capi.hip", "learned.hip", "mlp.hip"]
 ARCH = os.environ.get("AGIMP_OFFLOAD_ARCH", "gfx950")
 
 

@@ -53,6 +53,19 @@ struct agimp_handle {
   std::vector<hipEvent_t> event_pool;
   double phase_ms[AGIMP_NPHASES] = {};
   int64_t phase_calls[AGIMP_NPHASES] = {};
+  double mean_mass = 1.0;     // mean particle mass (reference density of the unit features)
+  // learned refinement prediction (agimp_learned_*)
+  int learn_on = 0;
+  int model_set = 0;
+  int window = 20;
+  LearnDesc U{};
+  int* cls_dev = nullptr;     // [nu] per-candidate classes (labels or predictions)
+  float* snap_f = nullptr;    // snapshot of the current particle buffer at a window start
+  int* snap_g = nullptr;
+  int* snap_id = nullptr;
+  unsigned long long* snap_cnt = nullptr;
+  Captured fixed[2];          // substep without the rebuild (bucketing + physics), per buffer index
+  int64_t lstats[AGIMP_NLSTATS] = {};
 };
 
 struct agimp_batch {
@@ -242,6 +255,90 @@ hipError_t capture_substep(hipStream_t s, const SceneDesc* dd, const Dims& m, in
   return hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
 }
 
+// substep without the rebuild (bucketing + physics) of one scene, for reused octrees
+hipError_t capture_fixed(hipStream_t s, const SceneDesc* dd, const Dims& m, int par, Captured& cap) {
+  cap.release();
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  launch_bucket(s, dd, m, par);
+  launch_p2g(s, dd, m, 0.f, par);
+  launch_grid(s, dd, m, 0.f);
+  launch_g2p(s, dd, m, 0.f, par);
+  hipError_t le = hipGetLastError();
+  e = hipStreamEndCapture(s, &cap.graph);
+  if (le != hipSuccess) return le;
+  if (e != hipSuccess) return e;
+  return hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
+}
+
+// dt of graph replays lives in the device descriptor
+int set_graph_dt(agimp_handle* h, float dt) {
+  if (dt == h->graph_dt) return 0;
+  h->hd.ph.dt = dt;
+  int rc = upload_desc(h);
+  if (rc) return rc;
+  h->graph_dt = dt;
+  return 0;
+}
+
+int learn_ready(agimp_handle* h) {
+  int rc = check_ready(h);
+  if (rc) return rc;
+  if (!h->learn_on) return fail(h, AGIMP_ESTATE, "learned prediction not enabled (agimp_learned_enable)");
+  return 0;
+}
+
+// candidates' features of the current particles (device only)
+int learn_features(agimp_handle* h, float dt) {
+  h->U.win = (float)h->window * dt;
+  launch_unit_features(h->stream, h->dd, h->D, h->n, h->host_parity, h->U);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int learn_count(agimp_handle* h, int which, int64_t* out) {
+  int c = 0;
+  HIPCHK(h, hipMemcpyAsync(&c, h->U.counts + which, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  *out = c;
+  return 0;
+}
+
+// predicted octree: features -> network -> refine with the predicted units' cells
+int learn_rebuild(agimp_handle* h, float dt) {
+  PhaseScope ps(h, AGIMP_PHASE_REFINE);
+  int rc = learn_features(h, dt);
+  if (rc) return rc;
+  launch_unit_predict(h->stream, h->U, h->L, nullptr);
+  launch_refine(h->stream, h->dd, h->dims, h->host_parity, &h->U);
+  h->topo_valid = 1;
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+size_t particle_floats(const agimp_handle* h) {
+  const int NF = (h->D == 2) ? PLayout<2>::NF : PLayout<3>::NF;
+  return (size_t)NF * (size_t)h->hd.P[0].stride;
+}
+
+int snapshot(agimp_handle* h, int par, bool save) {
+  const PBuf& P = h->hd.P[par];
+  const size_t nf = particle_floats(h) * sizeof(float), ni = (size_t)P.stride * sizeof(int);
+  const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+  if (save) {
+    HIPCHK(h, hipMemcpyAsync(h->snap_f, P.f, nf, k, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->snap_g, P.g, ni, k, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->snap_id, P.id, ni, k, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->snap_cnt, h->hd.cnt, NCOUNTERS * 8, k, h->stream));
+  } else {
+    HIPCHK(h, hipMemcpyAsync(P.f, h->snap_f, nf, k, h->stream));
+    HIPCHK(h, hipMemcpyAsync(P.g, h->snap_g, ni, k, h->stream));
+    HIPCHK(h, hipMemcpyAsync(P.id, h->snap_id, ni, k, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->hd.cnt, h->snap_cnt, NCOUNTERS * 8, k, h->stream));
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -388,6 +485,8 @@ int agimp_destroy(agimp_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   h->graph[0].release();
   h->graph[1].release();
+  h->fixed[0].release();
+  h->fixed[1].release();
   for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
   for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
@@ -431,6 +530,9 @@ int agimp_set_particles(agimp_handle* h, const float* x, const float* v, const f
   if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("set_particles: ") + hipGetErrorString(e));
+  double msum = 0.0;
+  for (int i = 0; i < n; ++i) msum += m[i];
+  h->mean_mass = msum > 0.0 ? msum / n : 1.0;
   h->has_particles = 1;
   return 0;
 }
@@ -504,12 +606,8 @@ int agimp_advance(agimp_handle* h, float dt, int32_t n) {
     return 0;
   }
   // replay one captured substep n times; dt lives in the device descriptor
-  if (dt != h->graph_dt) {
-    h->hd.ph.dt = dt;
-    rc = upload_desc(h);
-    if (rc) return rc;
-    h->graph_dt = dt;
-  }
+  rc = set_graph_dt(h, dt);
+  if (rc) return rc;
   for (int p = 0; p < 2; ++p) {
     if (h->graph[p].exec) continue;
     hipError_t e = capture_substep(h->stream, h->dd, h->dims, p, h->graph[p]);
@@ -639,6 +737,216 @@ int agimp_sync(agimp_handle* h) {
 }
 
 const char* agimp_last_error(agimp_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+// ------------------------------------------------------------------------------ learned prediction
+
+int agimp_learned_enable(agimp_handle* h, int32_t window, float tau) {
+  if (!h || window < 1 || !(tau > 0.f && tau <= 1.f)) return AGIMP_EINVAL;
+  if (h->hd.static_mode) return fail(h, AGIMP_ESTATE, "learned prediction needs dynamic adaptivity");
+  h->window = window;
+  h->U.tau = tau;
+  if (!h->learn_on) {
+    LearnDesc& U = h->U;
+    U.ulev = std::max(0, h->L - 3);
+    const LevelDesc& Lu = h->hd.T.lv[U.ulev];
+    const int p4 = h->D == 2 ? 16 : 64;
+    U.nu = Lu.nblocks * p4;
+    const size_t nu = (size_t)U.nu, nu4 = (nu + 3) & ~(size_t)3;
+    int rc = 0;
+    rc = rc ? rc : dalloc(h, &U.acc, nu * UNIT_ACC);
+    rc = rc ? rc : dalloc(h, &U.uflag, nu4);
+    rc = rc ? rc : dalloc(h, &U.cflag, nu4);
+    rc = rc ? rc : dalloc(h, &U.upred, nu4);
+    rc = rc ? rc : dalloc(h, &U.occ, nu);
+    rc = rc ? rc : dalloc(h, &U.cand, nu);
+    rc = rc ? rc : dalloc(h, &U.counts, 4);
+    rc = rc ? rc : dalloc(h, &U.feat, nu * UNIT_NFEAT);
+    rc = rc ? rc : dalloc(h, &U.lab, nu);
+    float* params = nullptr;
+    rc = rc ? rc : dalloc(h, &params, (size_t)AGIMP_MLP_NPARAMS);
+    rc = rc ? rc : dalloc(h, &h->cls_dev, nu);
+    rc = rc ? rc : dalloc(h, &h->snap_f, particle_floats(h));
+    rc = rc ? rc : dalloc(h, &h->snap_g, (size_t)h->hd.P[0].stride);
+    rc = rc ? rc : dalloc(h, &h->snap_id, (size_t)h->hd.P[0].stride);
+    rc = rc ? rc : dalloc(h, &h->snap_cnt, NCOUNTERS);
+    if (rc) return rc;
+    HIPCHK(h, hipMemsetAsync(U.upred, 0xff, nu4, h->stream));
+    U.params = params;
+    double vol = 1.0;
+    for (int a = 0; a < h->D; ++a) vol *= (double)Lu.dx;
+    U.inv_mref = (float)((double)h->cfg.p_vol / (h->mean_mass * vol));
+    h->hd.check = 1;
+    h->hd.ulev = U.ulev;
+    h->hd.upred = U.upred;
+    int rc2 = upload_desc(h);
+    if (rc2) return rc2;
+    h->learn_on = 1;
+  }
+  return 0;
+}
+
+int agimp_learned_set_model(agimp_handle* h, const float* params, int32_t n) {
+  if (!h || !params || n != AGIMP_MLP_NPARAMS) return AGIMP_EINVAL;
+  if (!h->learn_on) return fail(h, AGIMP_ESTATE, "learned prediction not enabled (agimp_learned_enable)");
+  HIPCHK(h, hipMemcpyAsync((void*)h->U.params, params, sizeof(float) * n, hipMemcpyHostToDevice, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->model_set = 1;
+  return 0;
+}
+
+int agimp_learned_features(agimp_handle* h, float dt, float* feat, int32_t* units, int64_t cap, int64_t* count) {
+  if (cap < 0 || !count) return AGIMP_EINVAL;
+  int rc = learn_ready(h);
+  if (rc) return rc;
+  rc = learn_features(h, dt);
+  if (rc) return rc;
+  int64_t nc = 0;
+  rc = learn_count(h, 1, &nc);
+  if (rc) return rc;
+  *count = nc;
+  const int64_t k = std::min(nc, cap);
+  if (k > 0 && feat)
+    HIPCHK(h, hipMemcpyAsync(feat, h->U.feat, (size_t)k * UNIT_NFEAT * 4, hipMemcpyDeviceToHost, h->stream));
+  if (k > 0 && units) HIPCHK(h, hipMemcpyAsync(units, h->U.cand, (size_t)k * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int agimp_learned_labels_reset(agimp_handle* h) {
+  int rc = learn_ready(h);
+  if (rc) return rc;
+  launch_unit_labels_clear(h->stream, h->U);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int agimp_learned_labels_accumulate(agimp_handle* h) {
+  int rc = learn_ready(h);
+  if (rc) return rc;
+  launch_unit_labels_accumulate(h->stream, h->dd, h->D, h->n, h->host_parity, h->U);
+  HIPCHK(h, hipGetLastError());
+  return 0;
+}
+
+int agimp_learned_labels(agimp_handle* h, int32_t* cls, int64_t cap) {
+  if (!cls || cap < 0) return AGIMP_EINVAL;
+  int rc = learn_ready(h);
+  if (rc) return rc;
+  int64_t nc = 0;
+  rc = learn_count(h, 1, &nc);
+  if (rc) return rc;
+  launch_unit_label_classes(h->stream, h->U, h->L, h->cls_dev);
+  HIPCHK(h, hipGetLastError());
+  const int64_t k = std::min(nc, cap);
+  if (k > 0) HIPCHK(h, hipMemcpyAsync(cls, h->cls_dev, (size_t)k * 4, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int agimp_learned_predict(agimp_handle* h, float dt, int32_t* cls, int8_t* level, int64_t cap, int64_t* count) {
+  if (cap < 0 || !count) return AGIMP_EINVAL;
+  int rc = learn_ready(h);
+  if (rc) return rc;
+  if (!h->model_set) return fail(h, AGIMP_ESTATE, "no model (agimp_learned_set_model)");
+  rc = learn_features(h, dt);
+  if (rc) return rc;
+  launch_unit_predict(h->stream, h->U, h->L, h->cls_dev);
+  HIPCHK(h, hipGetLastError());
+  int64_t nc = 0;
+  rc = learn_count(h, 1, &nc);
+  if (rc) return rc;
+  *count = nc;
+  const int64_t k = std::min(nc, cap);
+  if (k > 0 && cls) HIPCHK(h, hipMemcpyAsync(cls, h->cls_dev, (size_t)k * 4, hipMemcpyDeviceToHost, h->stream));
+  if (level) HIPCHK(h, hipMemcpyAsync(level, h->U.upred, (size_t)h->U.nu, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int agimp_learned_advance(agimp_handle* h, float dt, int32_t n) {
+  if (n < 0) return AGIMP_EINVAL;
+  int rc = learn_ready(h);
+  if (rc) return rc;
+  if (!h->model_set) return fail(h, AGIMP_ESTATE, "no model (agimp_learned_set_model)");
+  if (n == 0) return 0;
+  rc = set_graph_dt(h, dt);
+  if (rc) return rc;
+  for (int p = 0; p < 2; ++p) {
+    if (h->fixed[p].exec) continue;
+    hipError_t e = capture_fixed(h->stream, h->dd, h->dims, p, h->fixed[p]);
+    if (e != hipSuccess) {
+      h->fixed[p].release();
+      return fail(h, AGIMP_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+  }
+  const unsigned long long win_init[2] = {~0ull, 0ull};  // CNT_FIRSTVIOL, CNT_STEP
+  int t = 0;
+  while (t < n) {
+    rc = learn_rebuild(h, dt);
+    if (rc) return rc;
+    h->lstats[AGIMP_LSTAT_REBUILDS] += 1;
+    const int w = std::min(h->window, n - t);
+    const int par0 = h->host_parity;
+    rc = snapshot(h, par0, true);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(h->hd.cnt + CNT_FIRSTVIOL, win_init, sizeof(win_init), hipMemcpyHostToDevice, h->stream));
+    for (int i = 0; i < w; ++i) {
+      HIPCHK(h, hipGraphLaunch(h->fixed[h->host_parity].exec, h->stream));
+      h->host_parity ^= 1;
+    }
+    unsigned long long st[3], v0 = 0;  // CNT_VIOL, CNT_FIRSTVIOL, CNT_STEP; CNT_VIOL at the window start
+    HIPCHK(h, hipMemcpyAsync(st, h->hd.cnt + CNT_VIOL, sizeof(st), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&v0, h->snap_cnt + CNT_VIOL, sizeof(v0), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (st[1] <= (unsigned long long)w) {
+      // the check failed before substep st[1] of the window: keep the st[1]-1 substeps that
+      // ran on a valid octree, then rebuild from there
+      h->lstats[AGIMP_LSTAT_VIOLATIONS] += (int64_t)(st[0] - v0);
+      const int ok = (int)st[1] - 1;
+      rc = snapshot(h, par0, false);
+      if (rc) return rc;
+      h->host_parity = par0;
+      for (int i = 0; i < ok; ++i) {
+        HIPCHK(h, hipGraphLaunch(h->fixed[h->host_parity].exec, h->stream));
+        h->host_parity ^= 1;
+      }
+      h->lstats[AGIMP_LSTAT_FORCED] += 1;
+      h->lstats[AGIMP_LSTAT_DISCARDED] += w;
+      h->lstats[AGIMP_LSTAT_REPLAYED] += ok;
+      t += ok;
+      if (ok == 0) {  // not reachable: the octree was just built from these particles
+        rc = do_refine(h);
+        if (rc) return rc;
+        rc = do_physics(h, dt);
+        if (rc) return rc;
+        t += 1;
+      }
+    } else {
+      t += w;
+    }
+  }
+  h->lstats[AGIMP_LSTAT_SUBSTEPS] += n;
+  return 0;
+}
+
+int agimp_learned_stats(agimp_handle* h, int64_t* out, int32_t n) {
+  if (!h || !out || n < 0) return AGIMP_EINVAL;
+  if (h->learn_on) {
+    int c[4] = {0, 0, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(c, h->U.counts, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->lstats[AGIMP_LSTAT_CANDIDATES] = c[1];
+    h->lstats[AGIMP_LSTAT_REFINED] = c[2];
+  }
+  for (int i = 0; i < n && i < AGIMP_NLSTATS; ++i) out[i] = h->lstats[i];
+  return 0;
+}
+
+int agimp_learned_reset_stats(agimp_handle* h) {
+  if (!h) return AGIMP_EINVAL;
+  for (int i = 0; i < AGIMP_NLSTATS; ++i) h->lstats[i] = 0;
+  return 0;
+}
 
 // ------------------------------------------------------------------------------ batches
 

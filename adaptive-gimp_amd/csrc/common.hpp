@@ -116,6 +116,12 @@ struct SceneDesc {
   unsigned long long* cnt;
   uint32_t* vflag;
   size_t w_cells, w_node, w_flags;  // 32-bit words of the cell masks, node mask, block flags
+  // refinement criterion check (used while a predicted octree is reused): a particle's 2^d
+  // request cells at level g_p must be covered by cells of level >= g_p, i.e. hold the
+  // ACTIVATED bit, or lie in a unit (cell of level ulev) whose predicted level upred >= g_p
+  int check;
+  int ulev;
+  const int8_t* upred;
 };
 
 // counters kept on the device (int64 each)
@@ -125,7 +131,37 @@ enum Counter : int {
   CNT_ACTIVE = 2,     // particles transferred this substep
   CNT_CHUNKS = 3,     // P2G work chunks
   CNT_NAN = 4,        // particles whose new state is not finite
+  CNT_VIOL = 5,       // particles whose refinement criterion the octree failed (check on)
+  CNT_FIRSTVIOL = 6,  // smallest substep number (CNT_STEP) with a failed check
+  CNT_STEP = 7,       // substeps started (incremented by the bucketing phase)
   NCOUNTERS = 8
+};
+
+// Units of the learned refinement prediction: the cells of level ulev = max(0, L-3), i.e.
+// blocks of 4^d finest cells, in the padded block-major layout of that level.
+constexpr int UNIT_ACC = 16;  // accumulator words per unit
+enum UnitAcc : int {
+  UA_MASS = 0, UA_COUNT = 1, UA_VSUM = 2, UA_VMAX = 3, UA_CSUM = 4, UA_JSUM = 5, UA_JMIN = 6,  // JMIN: max key of -det F
+  UA_GMAX = 7, UA_GSUM = 8, UA_V = 9, UA_REQ = 12  // GMAX, REQ: level + 1 (0: none); V: 3 words
+};
+constexpr int UNIT_NFEAT = 16;
+
+struct LearnDesc {
+  int ulev;            // unit level
+  int nu;              // units (cells of level ulev incl. padding)
+  float win;           // window length in seconds (velocities -> displacements)
+  float inv_mref;      // 1 / (reference density * unit volume)
+  float tau;           // decision threshold on the cumulative class probability
+  uint32_t* acc;       // [nu * UNIT_ACC]
+  uint8_t* uflag;      // [nu] unit holds particles
+  uint8_t* cflag;      // [nu] unit is a candidate (within one unit of an occupied one)
+  int* occ;            // [nu] compacted occupied units
+  int* cand;           // [nu] compacted candidates
+  int* counts;         // [4] occupied, candidates, units refined by the prediction, -
+  float* feat;         // [nu * UNIT_NFEAT] features of the candidates (candidate order)
+  int* lab;            // [nu] finest requested level + 1 over a label window
+  int8_t* upred;       // [nu] predicted level (-1: none)
+  const float* params; // network parameters (5508 floats)
 };
 
 // block-major index of padded coordinates J (each in [0, E))

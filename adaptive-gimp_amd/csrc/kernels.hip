@@ -144,6 +144,7 @@ __global__ void k_clear_bucket(const SceneDesc* __restrict__ Dsc) {
   fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
   fill_zero((uint32_t*)sd.S.frozen, 1);
   fill_zero((uint32_t*)T.scount, MAXL);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
 }
 
 // the valid-node block flags of the octree: save after refine (to_saved) or restore
@@ -491,6 +492,17 @@ __global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
   }
 }
 
+// level-l cell I lies in a unit (cell of level ulev <= l) whose predicted level is >= l
+template <int D>
+__device__ __forceinline__ bool unit_covers(const Topo& T, int ulev, const int8_t* upred, int l, const int* I) {
+  if (!upred || l < ulev) return false;
+  int Iu[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) Iu[a] = I[a] >> (l - ulev);
+  const int64_t q = lidx<D>(T.lv[ulev], Iu);
+  return q >= 0 && (int)upred[q] >= l;
+}
+
 // ------------------------------------------------------------------------------ bucketing
 // adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
 // particle's 2^d cells; rank inside its (level, leaf block) bucket by run-aggregated atomics.
@@ -512,6 +524,8 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
     float x[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
+    const int gp = sd.check ? P.g[k] : -1;  // level whose request cells are checked
+    bool viol = false;
     for (int l = 0; l < T.L; ++l) {
       const LevelDesc& Ll = T.lv[l];
       int b[3];
@@ -524,9 +538,15 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
 #pragma unroll
         for (int a = 0; a < D; ++a) I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
         int64_t q = lidx<D>(Ll, I);
-        hit = hit || (q >= 0 && Ll.cell[q] == ACTIVATED);
+        const uint8_t c = (q >= 0) ? Ll.cell[q] : (uint8_t)0;
+        hit = hit || c == ACTIVATED;
+        if (l == gp && q >= 0 && !(c & ACTIVATED)) viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
       }
       if (hit) fl = l;
+    }
+    if (viol) {
+      count(cnt, CNT_VIOL);
+      atomicMin(&cnt[CNT_FIRSTVIOL], cnt[CNT_STEP]);
     }
     if (fl >= 0) {
       const LevelDesc& Lf = T.lv[fl];
@@ -1394,7 +1414,7 @@ static inline dim3 grid2(size_t x, int nscenes) {
   return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(x, 1u << 20)), (unsigned)nscenes);
 }
 
-void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 bg = grid2(blocks_for(m.max_total_blocks, 256), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
@@ -1405,6 +1425,7 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
     if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
     if (m.any_dynamic) {
       hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
+      if (learn) launch_unit_activate(s, d, m.dim, *learn);
       if (m.maxL > 1) {
         hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 0);
         hipLaunchKernelGGL(k_mark_hier<DD>, lg, dim3(256), 0, s, d);

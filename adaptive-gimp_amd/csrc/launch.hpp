@@ -22,7 +22,9 @@ struct Dims {
 
 size_t cells_total(const Topo& T, int lmin, int lmax);
 // par: index of the current particle buffer of every scene of the launch (G2P writes par ^ 1)
-void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par);
+// learn (optional, single scene): cells of the predicted units are activated after the
+// particles' own requests, before the culling
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn = nullptr);
 void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par);
 void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt);
@@ -34,5 +36,13 @@ void launch_download(hipStream_t s, const SceneDesc* d, int par, int dim, int n,
                      float* C, float* Jp, int* fl, int* g);
 void launch_dense_level(hipStream_t s, const SceneDesc* d, const Topo& T, int l, int* cell, float* m, float* v);
 void launch_dense_node(hipStream_t s, const SceneDesc* d, const Topo& T, int* out);
+
+// learned refinement prediction (learned.hip), single scene
+void launch_unit_features(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U);
+void launch_unit_predict(hipStream_t s, const LearnDesc& U, int L, int* cls_out);
+void launch_unit_activate(hipStream_t s, const SceneDesc* d, int dim, const LearnDesc& U);
+void launch_unit_labels_clear(hipStream_t s, const LearnDesc& U);
+void launch_unit_labels_accumulate(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U);
+void launch_unit_label_classes(hipStream_t s, const LearnDesc& U, int L, int* out);
 
 }  // namespace agimp

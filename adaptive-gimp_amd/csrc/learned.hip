@@ -1,0 +1,474 @@
+// Learned refinement prediction: per-unit features from the particles, window labels from
+// the plain criterion, the fused network + decision, and the activation of predicted cells.
+// None of this has a counterpart in the reference, whose octree is rebuilt from the current
+// particles every substep (adaptive_gimp.py:87-96, :323-337).
+//
+// A unit is a cell of level ulev = max(0, L-3): a block of 4^d finest cells.  Features are
+// accumulated per unit with segmented wavefront reductions (particles are stored sorted by
+// (level, leaf block, cell), so lanes of one unit are mostly contiguous) and one atomic per
+// run; candidates are the units within one unit of an occupied one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "common.hpp"
+#include "launch.hpp"
+#include "mlp.hpp"
+
+namespace agimp {
+
+template <int D>
+struct P4 { static constexpr int v = (D == 2) ? 16 : 64; };
+
+template <int D>
+__device__ __forceinline__ int64_t unit_lidx(const LevelDesc& L, const int* I) {
+  int J[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) J[a] = I[a] + PAD;
+  if (!in_box<D>(L.E, J)) return -1;
+  return (int64_t)bidx<D>(L.NB, J);
+}
+
+// padded coordinates of block-major cell index q
+template <int D>
+__device__ __forceinline__ void unit_coords(const LevelDesc& L, int q, int* J) {
+  const int b = q / P4<D>::v, w = q % P4<D>::v;
+  int B[3];
+  block_coords<D>(L.NB, b, B);
+  if (D == 2) {
+    J[0] = B[0] * 4 + (w >> 2);
+    J[1] = B[1] * 4 + (w & 3);
+  } else {
+    J[0] = B[0] * 4 + (w >> 4);
+    J[1] = B[1] * 4 + ((w >> 2) & 3);
+    J[2] = B[2] * 4 + (w & 3);
+  }
+}
+
+// monotone map of floats to unsigned keys; zero-initialised memory is below every key
+__device__ __forceinline__ uint32_t fkey(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float funkey(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int udpp(int src, int old) {
+  return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWMASK, 0xf, false);
+}
+
+// lanes sorted by key form segments; a step combines only with a source lane of the same key
+struct USeg {
+  bool s1, s2, s4, s8, b15, b31, tail;
+};
+__device__ __forceinline__ USeg useg(int key) {
+  USeg m;
+  m.s1 = udpp<0x111, 0xf>(key, -1) == key;
+  m.s2 = udpp<0x112, 0xf>(key, -1) == key;
+  m.s4 = udpp<0x114, 0xf>(key, -1) == key;
+  m.s8 = udpp<0x118, 0xf>(key, -1) == key;
+  m.b15 = udpp<0x142, 0xa>(key, -1) == key;
+  m.b31 = udpp<0x143, 0xc>(key, -1) == key;
+  m.tail = (__lane_id() == 63) || (__shfl_down(key, 1) != key);
+  return m;
+}
+#define USEG_STEP(OP, v, m)                                                                               \
+  do {                                                                                                   \
+    float t_;                                                                                            \
+    t_ = __int_as_float(udpp<0x111, 0xf>(__float_as_int(v), __float_as_int(v))); if (m.s1) v = OP(v, t_);  \
+    t_ = __int_as_float(udpp<0x112, 0xf>(__float_as_int(v), __float_as_int(v))); if (m.s2) v = OP(v, t_);  \
+    t_ = __int_as_float(udpp<0x114, 0xf>(__float_as_int(v), __float_as_int(v))); if (m.s4) v = OP(v, t_);  \
+    t_ = __int_as_float(udpp<0x118, 0xf>(__float_as_int(v), __float_as_int(v))); if (m.s8) v = OP(v, t_);  \
+    t_ = __int_as_float(udpp<0x142, 0xa>(__float_as_int(v), __float_as_int(v))); if (m.b15) v = OP(v, t_); \
+    t_ = __int_as_float(udpp<0x143, 0xc>(__float_as_int(v), __float_as_int(v))); if (m.b31) v = OP(v, t_); \
+  } while (0)
+__device__ __forceinline__ float op_add(float a, float b) { return a + b; }
+__device__ __forceinline__ float op_max(float a, float b) { return fmaxf(a, b); }
+
+// the particle's requests (adaptive_gimp.py:87-96): its 2^d cells at level g; record g+1 in
+// the units they lie in, for g >= ulev (coarser requests are not predicted)
+template <int D>
+__device__ __forceinline__ void record_requests(const Topo& T, int ulev, const float* x, int g, int* dst,
+                                                int stride) {
+  if (g < ulev || g >= T.L) return;
+  const LevelDesc& Lg = T.lv[g];
+  const LevelDesc& Lu = T.lv[ulev];
+  int b[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Lg, T.pow2);
+  int64_t seen[1 << D];
+#pragma unroll
+  for (int t = 0; t < (1 << D); ++t) {
+    int Iu[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) Iu[a] = (b[a] + ((t >> (D - 1 - a)) & 1)) >> (g - ulev);
+    const int64_t q = unit_lidx<D>(Lu, Iu);
+    seen[t] = q;
+    bool dup = q < 0;
+#pragma unroll
+    for (int s = 0; s < t; ++s) dup = dup || seen[s] == q;
+    if (!dup) atomicMax(&dst[q * stride], g + 1);
+  }
+}
+
+// ------------------------------------------------------------------------------ clears
+// accumulators of the previous candidates (the only units ever written)
+__global__ void k_unit_clear_acc(LearnDesc U) {
+  const int nc = U.counts[1];
+  const int64_t tot = (int64_t)nc * UNIT_ACC;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x)
+    U.acc[(int64_t)U.cand[i / UNIT_ACC] * UNIT_ACC + i % UNIT_ACC] = 0u;
+}
+
+// flags, predicted levels (-1) and counters
+__global__ void k_unit_clear_flags(LearnDesc U) {
+  const int64_t nw = (U.nu + 3) / 4;
+  uint32_t* uf = reinterpret_cast<uint32_t*>(U.uflag);
+  uint32_t* cf = reinterpret_cast<uint32_t*>(U.cflag);
+  uint32_t* up = reinterpret_cast<uint32_t*>(U.upred);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nw; i += (int64_t)gridDim.x * blockDim.x) {
+    uf[i] = 0u;
+    cf[i] = 0u;
+    up[i] = 0xffffffffu;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 4) U.counts[threadIdx.x] = 0;
+}
+
+__global__ void k_unit_clear_labels(LearnDesc U) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < U.nu; i += (int64_t)gridDim.x * blockDim.x)
+    U.lab[i] = 0;
+}
+
+// ------------------------------------------------------------------------------ accumulation
+template <int D>
+__global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, LearnDesc U) {
+  const SceneDesc& sd = Dsc[0];
+  const Topo& T = sd.T;
+  const PBuf& P = sd.P[par];
+  using PL = PLayout<D>;
+  const int n = sd.n;
+  if (blockIdx.x * blockDim.x >= (unsigned)n) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = k < n;
+  const LevelDesc& Lu = T.lv[U.ulev];
+  int u = -1, g = -1;
+  float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
+  float m = 0.f, vn = 0.f, cn = 0.f, jm1 = 0.f, njac = 0.f;
+  if (live) {
+    int I[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      x[a] = P.f[(size_t)(PL::X + a) * P.stride + k];
+      v[a] = P.f[(size_t)(PL::V + a) * P.stride + k];
+      I[a] = (int)floorf(x[a] * Lu.inv_dx);
+    }
+    const int64_t q = unit_lidx<D>(Lu, I);
+    u = (int)q;
+    g = P.g[k];
+    m = P.f[(size_t)PL::M * P.stride + k];
+    float F[3][3], C2 = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int b = 0; b < D; ++b) {
+        F[a][b] = P.f[(size_t)(PL::F + a * D + b) * P.stride + k];
+        const float c = P.f[(size_t)(PL::C + a * D + b) * P.stride + k];
+        C2 += c * c;
+      }
+    float J;
+    if (D == 2) J = F[0][0] * F[1][1] - F[0][1] * F[1][0];
+    else
+      J = F[0][0] * (F[1][1] * F[2][2] - F[1][2] * F[2][1]) - F[0][1] * (F[1][0] * F[2][2] - F[1][2] * F[2][0]) +
+          F[0][2] * (F[1][0] * F[2][1] - F[1][1] * F[2][0]);
+#pragma unroll
+    for (int a = 0; a < D; ++a) vn += v[a] * v[a];
+    vn = sqrtf(vn);
+    cn = sqrtf(C2);
+    jm1 = J - 1.f;
+    njac = -J;
+    // request cells lie within one unit of the particle's unit: all are candidates
+    if (u >= 0) record_requests<D>(T, U.ulev, x, g, reinterpret_cast<int*>(U.acc) + UA_REQ, UNIT_ACC);
+  }
+  const bool in = live && u >= 0;
+  const int key = in ? u : -2 - (int)__lane_id();
+  const USeg sm = useg(key);
+  float s_m = m, s_n = in ? 1.f : 0.f, s_v = vn, s_c = cn, s_j = jm1, s_g = (float)g;
+  float s_v0 = v[0], s_v1 = v[1], s_v2 = v[2];
+  float x_v = vn, x_j = njac, x_g = (float)(g + 1);
+  USEG_STEP(op_add, s_m, sm);
+  USEG_STEP(op_add, s_n, sm);
+  USEG_STEP(op_add, s_v, sm);
+  USEG_STEP(op_add, s_c, sm);
+  USEG_STEP(op_add, s_j, sm);
+  USEG_STEP(op_add, s_g, sm);
+  USEG_STEP(op_add, s_v0, sm);
+  USEG_STEP(op_add, s_v1, sm);
+  if (D == 3) USEG_STEP(op_add, s_v2, sm);
+  USEG_STEP(op_max, x_v, sm);
+  USEG_STEP(op_max, x_j, sm);
+  USEG_STEP(op_max, x_g, sm);
+  if (in && sm.tail) {
+    float* af = reinterpret_cast<float*>(U.acc) + (int64_t)u * UNIT_ACC;
+    uint32_t* au = U.acc + (int64_t)u * UNIT_ACC;
+    atomicAdd(&af[UA_MASS], s_m);
+    atomicAdd(&af[UA_COUNT], s_n);
+    atomicAdd(&af[UA_VSUM], s_v);
+    atomicAdd(&af[UA_CSUM], s_c);
+    atomicAdd(&af[UA_JSUM], s_j);
+    atomicAdd(&af[UA_GSUM], s_g);
+    atomicAdd(&af[UA_V + 0], s_v0);
+    atomicAdd(&af[UA_V + 1], s_v1);
+    if (D == 3) atomicAdd(&af[UA_V + 2], s_v2);
+    atomicMax(&au[UA_VMAX], __float_as_uint(x_v));
+    atomicMax(&au[UA_JMIN], fkey(x_j));
+    atomicMax(reinterpret_cast<int*>(&au[UA_GMAX]), (int)x_g);
+    U.uflag[u] = 1;
+  }
+}
+
+// the window labels: finest requested level + 1 per unit, max over the calls since the clear
+template <int D>
+__global__ void k_unit_requests(const SceneDesc* __restrict__ Dsc, int par, LearnDesc U) {
+  const SceneDesc& sd = Dsc[0];
+  const PBuf& P = sd.P[par];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  float x[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
+  record_requests<D>(sd.T, U.ulev, x, P.g[k], U.lab, 1);
+}
+
+// ------------------------------------------------------------------------------ candidates
+// append the indices of set byte flags to list, count in *count
+__global__ void k_unit_compact(const uint8_t* __restrict__ flags, int n, int* list, int* count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = i < n && flags[i] != 0;
+  const int idx = wave_run_atomic_add(count, on);
+  if (on) list[idx] = i;
+}
+
+template <int D>
+__global__ void k_unit_dilate(const SceneDesc* __restrict__ Dsc, LearnDesc U) {
+  const LevelDesc& Lu = Dsc[0].T.lv[U.ulev];
+  const int no = U.counts[0];
+  constexpr int NN = (D == 2) ? 9 : 27;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)no * NN;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int q = U.occ[t / NN];
+    int r = (int)(t % NN);
+    int J[3];
+    unit_coords<D>(Lu, q, J);
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) { J[a] += r % 3 - 1; r /= 3; }
+    if (!in_box<D>(Lu.E, J)) continue;
+    U.cflag[bidx<D>(Lu.NB, J)] = 1;
+  }
+}
+
+// ------------------------------------------------------------------------------ features
+__device__ __forceinline__ float lev_code(int lp1, int L) {
+  // level + 1 (0: none) -> finest 1, one coarser 0.75, two coarser 0.5, coarser 0.25, none 0
+  if (lp1 <= 0) return 0.f;
+  return 0.25f * (float)(1 + max(0, 3 - (L - lp1)));
+}
+
+template <int D>
+__global__ void k_unit_features(const SceneDesc* __restrict__ Dsc, LearnDesc U) {
+  const Topo& T = Dsc[0].T;
+  const LevelDesc& Lu = T.lv[U.ulev];
+  const int L = T.L;
+  const int nc = U.counts[1];
+  const float disp = U.win * Lu.inv_dx;  // velocity -> displacement in units over the window
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    const int q = U.cand[i];
+    const uint32_t* au = U.acc + (int64_t)q * UNIT_ACC;
+    const float* af = reinterpret_cast<const float*>(au);
+    const float cnt = af[UA_COUNT];
+    const bool occ = cnt > 0.f;
+    const float ic = occ ? 1.f / cnt : 0.f;
+    float f[UNIT_NFEAT];
+    f[0] = af[UA_MASS] * U.inv_mref;
+    f[1] = af[UA_VSUM] * ic * disp;
+    f[2] = __uint_as_float(au[UA_VMAX]) * disp;
+    f[3] = af[UA_CSUM] * ic * U.win;
+    f[4] = af[UA_JSUM] * ic * 10.f;
+    f[5] = occ ? (-funkey(au[UA_JMIN]) - 1.f) * 10.f : 0.f;
+    f[6] = lev_code((int)au[UA_GMAX], L);
+    f[7] = occ ? lev_code((int)lrintf(af[UA_GSUM] * ic) + 1, L) : 0.f;
+    f[8] = lev_code((int)au[UA_REQ], L);
+    float vb[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < D; ++a) vb[a] = af[UA_V + a] * ic;
+    f[12] = vb[0] * disp;
+    f[13] = vb[1] * disp;
+    f[14] = (D == 3) ? vb[2] * disp : 0.f;
+    int J[3];
+    unit_coords<D>(Lu, q, J);
+    float nm = 0.f, ng = 0.f, nr = 0.f, up = 0.f;
+    constexpr int NN = (D == 2) ? 9 : 27;
+    for (int r0 = 0; r0 < NN; ++r0) {
+      if (r0 == NN / 2) continue;
+      int K[3], dd[3], r = r0;
+#pragma unroll
+      for (int a = D - 1; a >= 0; --a) { dd[a] = r % 3 - 1; K[a] = J[a] + dd[a]; r /= 3; }
+      if (!in_box<D>(Lu.E, K)) continue;
+      const int64_t qn = bidx<D>(Lu.NB, K);
+      const uint32_t* bu = U.acc + qn * UNIT_ACC;
+      const float* bf = reinterpret_cast<const float*>(bu);
+      nm += bf[UA_MASS];
+      const float gc = lev_code((int)bu[UA_GMAX], L);
+      ng = fmaxf(ng, gc);
+      nr = fmaxf(nr, lev_code((int)bu[UA_REQ], L));
+      const float cn = bf[UA_COUNT];
+      if (cn > 0.f) {
+        // the neighbour moves towards this unit: mean velocity . (this - neighbour) > 0
+        float dot = 0.f;
+#pragma unroll
+        for (int a = 0; a < D; ++a) dot -= bf[UA_V + a] * (float)dd[a];
+        if (dot > 0.f) up = fmaxf(up, gc);
+      }
+    }
+    f[9] = nm * U.inv_mref / (float)(NN - 1);
+    f[10] = ng;
+    f[11] = nr;
+    f[15] = up;
+    float4* out = reinterpret_cast<float4*>(U.feat + (int64_t)i * UNIT_NFEAT);
+    out[0] = make_float4(f[0], f[1], f[2], f[3]);
+    out[1] = make_float4(f[4], f[5], f[6], f[7]);
+    out[2] = make_float4(f[8], f[9], f[10], f[11]);
+    out[3] = make_float4(f[12], f[13], f[14], f[15]);
+  }
+}
+
+// class of each candidate's window label: 0 finest, 1 one coarser, 2 two coarser, 3 none
+__global__ void k_unit_label_classes(LearnDesc U, int L, int* out) {
+  const int nc = U.counts[1];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+    const int r = U.lab[U.cand[i]];
+    out[i] = (r <= 0) ? 3 : min(3, L - r);
+  }
+}
+
+// ------------------------------------------------------------------------------ prediction
+constexpr int PRED_T = 256;
+
+// network on the candidates' features, then the decision rule: class c = first whose
+// cumulative probability reaches tau (3 if none); predicted level L-1-c if >= ulev
+__global__ void __launch_bounds__(PRED_T) k_unit_predict(LearnDesc U, int L, int* cls_out) {
+  __shared__ __align__(16) MlpLds W;
+  mlp_load(W, U.params);
+  __syncthreads();
+  const int lane = __lane_id();
+  const int64_t rows = U.counts[1];
+  const int64_t ntiles = (rows + 15) / 16;
+  const int64_t wave = (int64_t)blockIdx.x * (PRED_T / 64) + threadIdx.x / 64;
+  const int64_t nwaves = (int64_t)gridDim.x * (PRED_T / 64);
+  for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
+    MlpTile t;
+    mlp_forward_tile(W, U.feat, rows, tile * 16, t);
+    const int64_t row = tile * 16 + lane;
+    int refined = 0;
+    if (lane < 16 && row < rows) {
+      const float mx = fmaxf(fmaxf(t.z[0], t.z[1]), fmaxf(t.z[2], t.z[3]));
+      float e[4], se = 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) { e[j] = __expf(t.z[j] - mx); se += e[j]; }
+      const float thr = U.tau * se;
+      int c = 3;
+      float cum = 0.f;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        cum += e[j];
+        if (c == 3 && cum >= thr) c = j;
+      }
+      const int lev = L - 1 - c;
+      const int8_t pl = (c < 3 && lev >= U.ulev) ? (int8_t)lev : (int8_t)-1;
+      U.upred[U.cand[row]] = pl;
+      refined = pl >= 0;
+      if (cls_out) cls_out[row] = c;
+    }
+    const unsigned long long b = __ballot(refined);
+    if (lane == 0 && b) atomicAdd(&U.counts[2], __popcll(b));
+  }
+}
+
+// cells of the predicted level inside every refined unit get ACTIVATED (between the
+// particle activation and the culling of the rebuild)
+template <int D>
+__global__ void k_unit_activate(const SceneDesc* __restrict__ Dsc, LearnDesc U) {
+  const Topo& T = Dsc[0].T;
+  const LevelDesc& Lu = T.lv[U.ulev];
+  const int nc = U.counts[1];
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)nc * P4<D>::v;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int q = U.cand[t / P4<D>::v];
+    const int w = (int)(t % P4<D>::v);
+    const int lev = U.upred[q];
+    if (lev < U.ulev) continue;
+    const int s = lev - U.ulev;  // 0..2: 2^s cells per axis
+    const int side = 1 << s;
+    int r = w, off[3];
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) { off[a] = r % side; r /= side; }
+    if (r != 0) continue;  // w >= side^d
+    int J[3], I[3];
+    unit_coords<D>(Lu, q, J);
+#pragma unroll
+    for (int a = 0; a < D; ++a) I[a] = (J[a] - PAD) * side + off[a];
+    const LevelDesc& Ll = T.lv[lev];
+    int Jl[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) Jl[a] = I[a] + PAD;
+    if (!in_box<D>(Ll.E, Jl)) continue;
+    const uint32_t cq = bidx<D>(Ll.NB, Jl);
+    Ll.cell[cq] = ACTIVATED;
+    T.cflag[Ll.gbase + (int)(cq / P4<D>::v)] = 1;
+  }
+}
+
+// ------------------------------------------------------------------------------ launchers
+#define LEARN_DIM(D, ...) \
+  do { if ((D) == 2) { constexpr int DD = 2; __VA_ARGS__; } else { constexpr int DD = 3; __VA_ARGS__; } } while (0)
+
+static inline unsigned nblk(int64_t n, unsigned t, unsigned cap) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + t - 1) / t, cap));
+}
+
+void launch_unit_features(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U) {
+  LEARN_DIM(dim, {
+    hipLaunchKernelGGL(k_unit_clear_acc, dim3(nblk((int64_t)U.nu * UNIT_ACC, 256, 2048)), dim3(256), 0, s, U);
+    hipLaunchKernelGGL(k_unit_clear_flags, dim3(nblk(((int64_t)U.nu + 3) / 4, 256, 2048)), dim3(256), 0, s, U);
+    hipLaunchKernelGGL(k_unit_accumulate<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U);
+    hipLaunchKernelGGL(k_unit_compact, dim3(nblk(U.nu, 256, 1u << 30)), dim3(256), 0, s, U.uflag, U.nu, U.occ,
+                       U.counts + 0);
+    hipLaunchKernelGGL(k_unit_dilate<DD>, dim3(2048), dim3(256), 0, s, d, U);
+    hipLaunchKernelGGL(k_unit_compact, dim3(nblk(U.nu, 256, 1u << 30)), dim3(256), 0, s, U.cflag, U.nu, U.cand,
+                       U.counts + 1);
+    hipLaunchKernelGGL(k_unit_features<DD>, dim3(1024), dim3(256), 0, s, d, U);
+  });
+}
+
+void launch_unit_predict(hipStream_t s, const LearnDesc& U, int L, int* cls_out) {
+  hipLaunchKernelGGL(k_unit_predict, dim3(512), dim3(PRED_T), 0, s, U, L, cls_out);
+}
+
+void launch_unit_activate(hipStream_t s, const SceneDesc* d, int dim, const LearnDesc& U) {
+  LEARN_DIM(dim, hipLaunchKernelGGL(k_unit_activate<DD>, dim3(2048), dim3(256), 0, s, d, U));
+}
+
+void launch_unit_labels_clear(hipStream_t s, const LearnDesc& U) {
+  hipLaunchKernelGGL(k_unit_clear_labels, dim3(nblk(U.nu, 256, 2048)), dim3(256), 0, s, U);
+}
+
+void launch_unit_labels_accumulate(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U) {
+  LEARN_DIM(dim, hipLaunchKernelGGL(k_unit_requests<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U));
+}
+
+void launch_unit_label_classes(hipStream_t s, const LearnDesc& U, int L, int* out) {
+  hipLaunchKernelGGL(k_unit_label_classes, dim3(1024), dim3(256), 0, s, U, L, out);
+}
+
+}  // namespace agimp

@@ -404,6 +404,64 @@ def c5_three_spheres(seed=0, n=4_000_000):
     return Workload("c5_3d_three_spheres_4M_4lv", 3, L, c, st, 2e-5, 2000, prm)
 
 
+def uniform_ball(center, radius, n, rng):
+    """n points uniform in a ball (used by the training scenes, which need many cheap draws)."""
+    d = len(center)
+    u = rng.normal(size=(n, d))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = radius * rng.random(n) ** (1.0 / d)
+    return np.asarray(center) + u * r[:, None]
+
+
+def training_scene(seed, n=200_000):
+    """Scenes of the kinds the learned refinement prediction is used on, for training data
+    (seeds 1 and up; the benchmark scenes use their own generators and seed 0):
+
+    * odd seeds: three spheres (L=4, coarsest 64) of radius U[0.07, 0.11] on a triangle around
+      (0.5, 0.5, 0.5), each moving to the centroid at U[1, 3] m/s, surfaces U[0.002, 0.05]
+      apart, shells r/R <= a0 < a1 < a2 with a0 ~ U[0.6, 0.8] (levels 0..3, finest outside);
+    * even seeds: one sphere (L=3, coarsest 64) of radius U[0.08, 0.16], U[0, 0.03] above a
+      sticky slab at y in [0.45, 0.55], falling at U[0, 3] m/s, shells r/R <= a0 < a1 (0..2).
+    Snow material of the benchmark scenes, rho = 400, n uniform particles."""
+    rng = np.random.default_rng(seed)
+    prm = dict(E=1.4e5, nu=0.2, gravity=(0.0, -9.8, 0.0), wall_mode=1, plastic=1, theta_c=2.5e-2,
+               theta_s=7.5e-3, hardening=10.0)
+    if seed % 2 == 1:
+        L, c = 4, 64
+        R = rng.uniform(0.07, 0.11)
+        gap = rng.uniform(0.002, 0.05)
+        rc = (2 * R + gap) / math.sqrt(3.0)  # centres on a circle: pairwise distance 2R + gap
+        phi0 = rng.uniform(0, 2 * math.pi)
+        a0 = rng.uniform(0.6, 0.8)
+        cuts = (a0, a0 + (1 - a0) * 0.5, a0 + (1 - a0) * 0.8)
+        xs, vs, gs = [], [], []
+        for k, nk in enumerate(np.diff(np.linspace(0, n, 4).astype(int))):
+            phi = phi0 + 2 * math.pi * k / 3
+            ck = np.array([0.5 + rc * math.cos(phi), 0.5 + 0.3 * rc * rng.uniform(-1, 1), 0.5 + rc * math.sin(phi)])
+            p = uniform_ball(ck, R, nk, rng)
+            d = np.array([0.5, 0.5, 0.5]) - ck
+            vs.append(np.tile(rng.uniform(1, 3) * d / np.linalg.norm(d), (nk, 1)))
+            rr = np.linalg.norm(p - ck, axis=1) / R
+            gs.append(np.searchsorted(np.asarray(cuts), rr).astype(np.int32))
+            xs.append(p)
+        x, v, g = np.concatenate(xs), np.concatenate(vs), np.concatenate(gs)
+        p_vol = 3 * (4.0 / 3.0) * math.pi * R ** 3 / n
+    else:
+        L, c = 3, 64
+        R = rng.uniform(0.08, 0.16)
+        ck = np.array([rng.uniform(0.3, 0.7), 0.55 + R + rng.uniform(0.0, 0.03), rng.uniform(0.3, 0.7)])
+        x = uniform_ball(ck, R, n, rng)
+        v = np.tile(np.array([0.0, -rng.uniform(0, 3), 0.0]), (n, 1))
+        a0 = rng.uniform(0.8, 0.92)
+        cuts = (a0, a0 + (1 - a0) * 0.8)
+        g = np.searchsorted(np.asarray(cuts), np.linalg.norm(x - ck, axis=1) / R).astype(np.int32)
+        p_vol = (4.0 / 3.0) * math.pi * R ** 3 / n
+        prm["slab"] = (1, 0.45, 0.55)
+    st = _state(x, v, g, 400.0 * p_vol, 3)
+    prm["p_vol"] = p_vol
+    return Workload(f"train_{seed}", 3, L, c, st, 2e-5, 200, prm)
+
+
 WORKLOADS = {
     "c1": c1_two_squares,
     "c2": c2_static_band,

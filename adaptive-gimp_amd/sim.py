@@ -154,6 +154,7 @@ class AdaptiveGIMP:
         }
         self._host_dirty = True
         self._device_newer = False
+        self._last_candidates = 0
         self._level_cache = None
         self._node_cache = None
         for name in self._host:
@@ -285,7 +286,81 @@ class AdaptiveGIMP:
         out = np.zeros(8, np.int64)
         _lib.check(self._L.agimp_get_counters(self._h, _lib.ptr(out, ctypes.c_int64), 8), self._h, "get_counters")
         return {"oob": int(out[0]), "frozen": int(out[1]), "active": int(out[2]), "chunks": int(out[3]),
-                "nonfinite": int(out[4])}
+                "nonfinite": int(out[4]), "criterion_failures": int(out[5])}
+
+    # ------------------------------------------------------------------ learned prediction
+    def learned_enable(self, window=20, tau=0.3):
+        """Allocate the refinement-prediction state; switches the criterion check on."""
+        _lib.check(self._L.agimp_learned_enable(self._h, int(window), float(tau)), self._h, "agimp_learned_enable")
+        self._learn_window = int(window)
+
+    def learned_set_model(self, params):
+        p = np.ascontiguousarray(params, np.float32).ravel()
+        if p.size != _lib.MLP_NPARAMS:
+            raise ValueError(f"expected {_lib.MLP_NPARAMS} parameters, got {p.size}")
+        _lib.check(self._L.agimp_learned_set_model(self._h, _lib.ptr(p, ctypes.c_float), p.size), self._h,
+                   "agimp_learned_set_model")
+
+    def learned_features(self, dt0):
+        """(features [k, 16] float32, unit ids [k] int32) of the candidate units now."""
+        self._push()
+        cap = self._learn_cap()
+        feat = np.empty((cap, 16), np.float32)
+        units = np.empty(cap, np.int32)
+        cnt = ctypes.c_int64(0)
+        _lib.check(self._L.agimp_learned_features(self._h, float(dt0), _lib.ptr(feat, ctypes.c_float),
+                                                  _lib.ptr(units, ctypes.c_int32), cap, ctypes.byref(cnt)),
+                   self._h, "agimp_learned_features")
+        k = min(cnt.value, cap)
+        self._last_candidates = cnt.value
+        return feat[:k].copy(), units[:k].copy()
+
+    def learned_labels_reset(self):
+        _lib.check(self._L.agimp_learned_labels_reset(self._h), self._h, "agimp_learned_labels_reset")
+
+    def learned_labels_accumulate(self):
+        self._push()
+        _lib.check(self._L.agimp_learned_labels_accumulate(self._h), self._h, "agimp_learned_labels_accumulate")
+
+    def learned_labels(self):
+        """Window classes (0 finest, 1, 2, 3 none) of the candidates of the last features call."""
+        k = self._last_candidates
+        out = np.empty(max(k, 1), np.int32)
+        _lib.check(self._L.agimp_learned_labels(self._h, _lib.ptr(out, ctypes.c_int32), k), self._h,
+                   "agimp_learned_labels")
+        return out[:k]
+
+    def learned_predict(self, dt0):
+        """(classes of the candidates, predicted level of every unit) on the current particles."""
+        self._push()
+        cap = self._learn_cap()
+        cls = np.empty(cap, np.int32)
+        lev = np.empty(cap, np.int8)
+        cnt = ctypes.c_int64(0)
+        _lib.check(self._L.agimp_learned_predict(self._h, float(dt0), _lib.ptr(cls, ctypes.c_int32),
+                                                 _lib.ptr(lev, ctypes.c_int8), cap, ctypes.byref(cnt)),
+                   self._h, "agimp_learned_predict")
+        return cls[:min(cnt.value, cap)].copy(), lev
+
+    def advance_learned(self, dt0, n):
+        """n substeps reusing predicted octrees, with the plain criterion checked every substep."""
+        self._push()
+        _lib.check(self._L.agimp_learned_advance(self._h, float(dt0), int(n)), self._h, "agimp_learned_advance")
+        self._invalidate()
+
+    def learned_stats(self):
+        out = np.zeros(len(_lib.LSTATS), np.int64)
+        _lib.check(self._L.agimp_learned_stats(self._h, _lib.ptr(out, ctypes.c_int64), len(out)), self._h,
+                   "agimp_learned_stats")
+        return {k: int(out[i]) for i, k in enumerate(_lib.LSTATS)}
+
+    def learned_reset_stats(self):
+        _lib.check(self._L.agimp_learned_reset_stats(self._h), self._h, "agimp_learned_reset_stats")
+
+    def _learn_cap(self):
+        ul = max(0, self.level - 3)
+        nb = (self.extents[ul] + 8) // 4
+        return nb ** self.dim * (16 if self.dim == 2 else 64)
 
     # ------------------------------------------------------------------ reference helpers
     def activate_cell(self, l, I):

@@ -111,7 +111,9 @@ int agimp_get_node_mask(agimp_handle* h, int32_t* node_mask);
 
 /* Counters accumulated since creation (or the last reset): [0] out-of-box accesses,
  * [1] frozen particle-substeps (fl_p = -1), [2] transferred particle-substeps,
- * [3] P2G chunks, [4] non-finite particles. */
+ * [3] P2G chunks, [4] non-finite particles, [5] particles that failed the refinement
+ * criterion check (only counted once agimp_learned_enable has switched the check on),
+ * [6], [7] bookkeeping of the learned windows. */
 int agimp_get_counters(agimp_handle* h, int64_t* out, int32_t n);
 int agimp_reset_counters(agimp_handle* h);
 
@@ -144,6 +146,71 @@ int agimp_batch_destroy(agimp_batch* b);
 int agimp_batch_set_particles_per_chunk(agimp_batch* b, const int32_t* ch);
 int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps);
 const char* agimp_batch_last_error(agimp_batch* b);
+
+/* Learned refinement prediction (not in the reference, whose octree is rebuilt from the
+ * current particles every substep, adaptive_gimp.py:323-337).
+ *
+ * Units are the cells of level max(0, level-3), i.e. blocks of 4^dim finest cells.  For each
+ * unit within one unit of a particle-holding one ("candidates"), 16 features are computed
+ * from the particles (mass fraction, |v|, |grad v| = |C|, det F, level tags, neighbour mass
+ * and levels, mean displacement over the window, upwind level).  A 16-64-64-4 network
+ * (agimp_mlp_*) classifies each candidate by the finest level the plain criterion will
+ * request in it during the next `window` substeps: 0 finest, 1 one coarser, 2 two coarser,
+ * 3 none.  The predicted octree is the plain one of the current particles plus, in every
+ * unit of class c < 3, all cells of level (level-1-c) if that is not coarser than the unit.
+ *
+ * agimp_learned_advance reuses one predicted octree for up to `window` substeps (captured
+ * bucketing + physics replays).  Every substep checks the plain criterion in the level
+ * mapping: each particle's 2^dim request cells at level g_p (adaptive_gimp.py:87-96) must be
+ * covered at level >= g_p.  If a check fails, the window is rolled back to its start (a
+ * device snapshot of the particles), the substeps before the failing one are replayed, and
+ * the octree is rebuilt there: no substep runs on an octree coarser than the criterion
+ * requires. */
+#define AGIMP_MLP_NPARAMS 5508
+#define AGIMP_LSTAT_REBUILDS 0    /* predicted octrees built */
+#define AGIMP_LSTAT_FORCED 1      /* windows cut short by a failed criterion check */
+#define AGIMP_LSTAT_DISCARDED 2   /* substeps run and then rolled back */
+#define AGIMP_LSTAT_REPLAYED 3    /* substeps replayed after a rollback */
+#define AGIMP_LSTAT_SUBSTEPS 4    /* substeps advanced by agimp_learned_advance */
+#define AGIMP_LSTAT_VIOLATIONS 5  /* particle checks that failed (before the rollbacks) */
+#define AGIMP_LSTAT_CANDIDATES 6  /* candidate units of the last prediction */
+#define AGIMP_LSTAT_REFINED 7     /* units refined by the last prediction */
+#define AGIMP_NLSTATS 8
+
+/* Allocate the prediction state and switch the criterion check on; tau in (0, 1] is the
+ * decision threshold (class c = first whose cumulative probability reaches tau). */
+int agimp_learned_enable(agimp_handle* h, int32_t window, float tau);
+/* Network parameters (host, AGIMP_MLP_NPARAMS floats, layout of agimp_mlp_forward). */
+int agimp_learned_set_model(agimp_handle* h, const float* params, int32_t n);
+/* Candidates of the current particles: features [count][16] and unit ids (block-major cell
+ * index of the unit level); copies min(count, cap) rows, returns the total in *count. */
+int agimp_learned_features(agimp_handle* h, float dt, float* feat, int32_t* units, int64_t cap, int64_t* count);
+/* Window labels for training: reset, then accumulate the plain requests of the current
+ * particles before each of the window's substeps, then read the class of every candidate of
+ * the last agimp_learned_features call (0 finest .. 3 none). */
+int agimp_learned_labels_reset(agimp_handle* h);
+int agimp_learned_labels_accumulate(agimp_handle* h);
+int agimp_learned_labels(agimp_handle* h, int32_t* cls, int64_t cap);
+/* Features + network + decision on the current particles: classes of the candidates and,
+ * if level != NULL, the predicted level of every unit (int8, -1 none). */
+int agimp_learned_predict(agimp_handle* h, float dt, int32_t* cls, int8_t* level, int64_t cap, int64_t* count);
+/* n substeps with predicted octrees (see above). */
+int agimp_learned_advance(agimp_handle* h, float dt, int32_t n);
+int agimp_learned_stats(agimp_handle* h, int64_t* out, int32_t n);
+int agimp_learned_reset_stats(agimp_handle* h);
+
+/* The network on its own (device pointers, ordered on `stream`, NULL = default stream).
+ * Parameters: W1[64][16] b1[64] W2[64][64] b2[64] W3[4][64] b3[4] (row-major, FP32).
+ * Products run on BF16 matrix cores with FP32 accumulation: inputs, weights and hidden
+ * activations are rounded to BF16 before each product. */
+int agimp_mlp_forward(const float* X, int64_t rows, const float* params, float* logits, void* stream);
+/* loss = scale * sum_r w[y_r] * CE(softmax(z_r), y_r) and its gradient [5508]; w may be NULL
+ * (all 1).  grad and loss are overwritten. */
+int agimp_mlp_loss_grad(const float* X, const int32_t* y, int64_t rows, const float* params,
+                        const float* class_weight, float scale, float* grad, float* loss, void* stream);
+/* One Adam step (bias-corrected; decoupled weight decay on the weight matrices). */
+int agimp_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr, float beta1,
+                   float beta2, float eps, float weight_decay, void* stream);
 
 /* Host-side FP64 maths used by the kernels, exported for CPU checks: the rotation-variant
  * SVD (dim 2 or 3, row-major) and the constitutive update (clamp plasticity if enabled in
