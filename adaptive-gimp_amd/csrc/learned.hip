@@ -59,19 +59,28 @@ __device__ __forceinline__ int udpp(int src, int old) {
   return __builtin_amdgcn_update_dpp(old, src, CTRL, ROWMASK, 0xf, false);
 }
 
-// lanes sorted by key form segments; a step combines only with a source lane of the same key
+// Segmented inclusive scan over runs of equal keys.  A key may occur in several runs of one
+// wavefront (units interleave when particles are ordered by the cells of a coarser level),
+// so a step combines with its source lane only if that lane lies in the lane's own run
+// (source index >= the run's first lane) and the DPP move has a source at all (same key).
 struct USeg {
   bool s1, s2, s4, s8, b15, b31, tail;
 };
 __device__ __forceinline__ USeg useg(int key) {
   USeg m;
-  m.s1 = udpp<0x111, 0xf>(key, -1) == key;
-  m.s2 = udpp<0x112, 0xf>(key, -1) == key;
-  m.s4 = udpp<0x114, 0xf>(key, -1) == key;
-  m.s8 = udpp<0x118, 0xf>(key, -1) == key;
-  m.b15 = udpp<0x142, 0xa>(key, -1) == key;
-  m.b31 = udpp<0x143, 0xc>(key, -1) == key;
-  m.tail = (__lane_id() == 63) || (__shfl_down(key, 1) != key);
+  const int lane = __lane_id();
+  const int prev = __shfl_up(key, 1);
+  const unsigned long long heads = __ballot(lane == 0 || prev != key);
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const int head = 63 - __clzll(heads & upto);  // first lane of this lane's run
+  m.s1 = udpp<0x111, 0xf>(key, -1) == key && lane - 1 >= head;
+  m.s2 = udpp<0x112, 0xf>(key, -1) == key && lane - 2 >= head;
+  m.s4 = udpp<0x114, 0xf>(key, -1) == key && lane - 4 >= head;
+  m.s8 = udpp<0x118, 0xf>(key, -1) == key && lane - 8 >= head;
+  m.b15 = udpp<0x142, 0xa>(key, -1) == key && (lane & ~15) - 1 >= head;  // source: last lane of the previous row
+  m.b31 = udpp<0x143, 0xc>(key, -1) == key && 31 >= head;
+  const int next = __shfl_down(key, 1);  // every lane takes part in the shuffle (no short-circuit)
+  m.tail = (lane == 63) || (next != key);
   return m;
 }
 #define USEG_STEP(OP, v, m)                                                                               \
