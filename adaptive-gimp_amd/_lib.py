@@ -43,7 +43,10 @@ EXPORTS = [
     "agimp_host_svd", "agimp_host_constitutive",
     "agimp_learned_enable", "agimp_learned_set_model", "agimp_learned_features", "agimp_learned_labels_reset",
     "agimp_learned_labels_accumulate", "agimp_learned_labels", "agimp_learned_predict", "agimp_learned_advance",
-    "agimp_learned_stats", "agimp_learned_reset_stats", "agimp_mlp_forward", "agimp_mlp_loss_grad", "agimp_mlp_adam",
+    "agimp_learned_stats", "agimp_learned_reset_stats", "agimp_learned_set_parts", "agimp_mlp_forward", "agimp_mlp_loss_grad", "agimp_mlp_adam",
+    "agimp_trainer_create", "agimp_trainer_destroy", "agimp_trainer_set_data", "agimp_trainer_set_params",
+    "agimp_trainer_get_params", "agimp_trainer_epoch", "agimp_trainer_logits", "agimp_trainer_grad",
+    "agimp_trainer_apply",
 ]
 MLP_NPARAMS = 5508
 LSTATS = ["rebuilds", "forced", "discarded", "replayed", "substeps", "violations", "candidates", "refined"]
@@ -110,10 +113,20 @@ def load():
         "agimp_learned_advance": [vp, ctypes.c_float, ctypes.c_int32],
         "agimp_learned_stats": [vp, i64p, ctypes.c_int32],
         "agimp_learned_reset_stats": [vp],
+        "agimp_learned_set_parts": [vp, ctypes.c_int32],
         "agimp_mlp_forward": [vp, ctypes.c_int64, vp, vp, vp],
         "agimp_mlp_loss_grad": [vp, vp, ctypes.c_int64, vp, vp, ctypes.c_float, vp, vp, vp],
         "agimp_mlp_adam": [vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                            ctypes.c_float, ctypes.c_float, vp],
+        "agimp_trainer_create": [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(vp)],
+        "agimp_trainer_destroy": [vp],
+        "agimp_trainer_set_data": [vp, fp, ip, ctypes.c_int64],
+        "agimp_trainer_set_params": [vp, fp],
+        "agimp_trainer_get_params": [vp, fp],
+        "agimp_trainer_epoch": [vp, ip, ctypes.c_int64, fp, ctypes.c_float, ctypes.c_float, ctypes.c_float, dp],
+        "agimp_trainer_logits": [vp, fp, ctypes.c_int64, fp],
+        "agimp_trainer_grad": [vp, fp, ip, ctypes.c_int64, fp, ctypes.c_float, fp, fp],
+        "agimp_trainer_apply": [vp, fp, ctypes.c_float, ctypes.c_float],
     }
     for name, args in sig.items():
         f = getattr(L, name)

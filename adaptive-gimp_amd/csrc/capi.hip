@@ -745,6 +745,7 @@ int agimp_learned_enable(agimp_handle* h, int32_t window, float tau) {
   if (h->hd.static_mode) return fail(h, AGIMP_ESTATE, "learned prediction needs dynamic adaptivity");
   h->window = window;
   h->U.tau = tau;
+  if (!h->learn_on) h->U.mode = AGIMP_PREDICT_NETWORK | AGIMP_PREDICT_SWEEP;
   if (!h->learn_on) {
     LearnDesc& U = h->U;
     U.ulev = std::max(0, h->L - 3);
@@ -926,6 +927,13 @@ int agimp_learned_advance(agimp_handle* h, float dt, int32_t n) {
     }
   }
   h->lstats[AGIMP_LSTAT_SUBSTEPS] += n;
+  return 0;
+}
+
+int agimp_learned_set_parts(agimp_handle* h, int32_t parts) {
+  if (!h || parts < 0 || parts > (AGIMP_PREDICT_NETWORK | AGIMP_PREDICT_SWEEP)) return AGIMP_EINVAL;
+  if (!h->learn_on) return fail(h, AGIMP_ESTATE, "learned prediction not enabled (agimp_learned_enable)");
+  h->U.mode = parts;
   return 0;
 }
 

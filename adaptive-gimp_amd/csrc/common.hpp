@@ -19,6 +19,9 @@ constexpr int PAD = 4;   // padding cells on each side of every level (one leaf 
 
 // cell / node flag bits, utils.py:5-9
 constexpr uint8_t ACTIVATED = 1, GHOST = 2, T_JUNCTION = 4, CULLING = 8;
+// not in the reference: a cell without ACTIVATED whose whole volume is covered by finer
+// activated cells (set only by the rebuild of a predicted octree, read by the criterion check)
+constexpr uint8_t COVERED = 16;
 
 struct LevelDesc {
   int n;        // cells per axis in the domain, c * 2^l
@@ -118,7 +121,8 @@ struct SceneDesc {
   size_t w_cells, w_node, w_flags;  // 32-bit words of the cell masks, node mask, block flags
   // refinement criterion check (used while a predicted octree is reused): a particle's 2^d
   // request cells at level g_p must be covered by cells of level >= g_p, i.e. hold the
-  // ACTIVATED bit, or lie in a unit (cell of level ulev) whose predicted level upred >= g_p
+  // ACTIVATED or COVERED bit, or lie in a unit (cell of level ulev) whose predicted level
+  // upred >= g_p
   int check;
   int ulev;
   const int8_t* upred;
@@ -142,7 +146,8 @@ enum Counter : int {
 constexpr int UNIT_ACC = 16;  // accumulator words per unit
 enum UnitAcc : int {
   UA_MASS = 0, UA_COUNT = 1, UA_VSUM = 2, UA_VMAX = 3, UA_CSUM = 4, UA_JSUM = 5, UA_JMIN = 6,  // JMIN: max key of -det F
-  UA_GMAX = 7, UA_GSUM = 8, UA_V = 9, UA_REQ = 12  // GMAX, REQ: level + 1 (0: none); V: 3 words
+  UA_GMAX = 7, UA_GSUM = 8, UA_V = 9, UA_REQ = 12,  // GMAX, REQ: level + 1 (0: none); V: 3 words
+  UA_REQF = 13  // requests of the particles moved by v * window (level + 1)
 };
 constexpr int UNIT_NFEAT = 16;
 
@@ -152,6 +157,7 @@ struct LearnDesc {
   float win;           // window length in seconds (velocities -> displacements)
   float inv_mref;      // 1 / (reference density * unit volume)
   float tau;           // decision threshold on the cumulative class probability
+  int mode;            // parts of a predicted octree: 1 network unit levels, 2 swept requests
   uint32_t* acc;       // [nu * UNIT_ACC]
   uint8_t* uflag;      // [nu] unit holds particles
   uint8_t* cflag;      // [nu] unit is a candidate (within one unit of an occupied one)

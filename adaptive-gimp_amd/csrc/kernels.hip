@@ -492,6 +492,45 @@ __global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
   }
 }
 
+// Coverage of level-l cells by finer activated cells, level by level from the finest up (one
+// launch per l = L-2 .. 0): a level-l cell without ACTIVATED whose 2^d children are all
+// ACTIVATED or COVERED gets COVERED.  A parent's children lie in one leaf block of level l+1,
+// so each thread owns one parent cell (no races); blocks that receive COVERED are flagged so
+// the next coarser launch visits them.
+template <int D>
+__global__ void k_cover(const SceneDesc* __restrict__ Dsc, int l) {
+  AGIMP_SCENE
+  if (l + 1 >= T.L) return;
+  const LevelDesc& Lc = T.lv[l + 1];
+  const LevelDesc& Lp = T.lv[l];
+  const int64_t tot = (int64_t)Lc.nblocks << D;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(t >> D);
+    if (!T.cflag[Lc.gbase + b]) continue;
+    const int pb = (int)(t & ((1 << D) - 1));
+    int B[3], J[3];
+    block_coords<D>(Lc.NB, b, B);
+    bool all = true;
+#pragma unroll
+    for (int u = 0; u < (1 << D); ++u) {
+      int Jc[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) Jc[a] = 4 * B[a] + 2 * ((pb >> (D - 1 - a)) & 1) + ((u >> (D - 1 - a)) & 1);
+      all = all && (Lc.cell[bidx<D>(Lc.NB, Jc)] & (ACTIVATED | COVERED));
+    }
+    if (!all) continue;
+    // padded parent coordinates: (padded child - PAD)/2 + PAD
+#pragma unroll
+    for (int a = 0; a < D; ++a) J[a] = ((4 * B[a] + 2 * ((pb >> (D - 1 - a)) & 1) - PAD) >> 1) + PAD;
+    if (!in_box<D>(Lp.E, J)) continue;
+    const uint32_t q = bidx<D>(Lp.NB, J);
+    const uint8_t c = Lp.cell[q];
+    if (c & ACTIVATED) continue;
+    Lp.cell[q] = c | COVERED;
+    T.cflag[Lp.gbase + (int)(q / Pow4<D>::v)] = 1;
+  }
+}
+
 // level-l cell I lies in a unit (cell of level ulev <= l) whose predicted level is >= l
 template <int D>
 __device__ __forceinline__ bool unit_covers(const Topo& T, int ulev, const int8_t* upred, int l, const int* I) {
@@ -540,7 +579,8 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
         int64_t q = lidx<D>(Ll, I);
         const uint8_t c = (q >= 0) ? Ll.cell[q] : (uint8_t)0;
         hit = hit || c == ACTIVATED;
-        if (l == gp && q >= 0 && !(c & ACTIVATED)) viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
+        if (l == gp && q >= 0 && !(c & (ACTIVATED | COVERED)))
+          viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
       }
       if (hit) fl = l;
     }
@@ -1373,7 +1413,7 @@ __global__ void k_dense_level(const SceneDesc* __restrict__ Dsc, int l, int* cel
   size_t r = t;
   for (int a = D - 1; a >= 0; --a) { I[a] = (int)(r % Ll.e); r /= Ll.e; }
   int64_t q = lidx<D>(Ll, I);
-  uint8_t c = Ll.cell[q];
+  uint8_t c = Ll.cell[q] & (uint8_t)~COVERED;
   if (c & CULLING) c &= GHOST;
   cell[t] = c;
   float4 g = Ll.grid[q];
@@ -1425,7 +1465,7 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
     if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
     if (m.any_dynamic) {
       hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
-      if (learn) launch_unit_activate(s, d, m.dim, *learn);
+      if (learn) launch_unit_activate(s, d, m.dim, (int)m.max_n, par, *learn);
       if (m.maxL > 1) {
         hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 0);
         hipLaunchKernelGGL(k_mark_hier<DD>, lg, dim3(256), 0, s, d);
@@ -1438,6 +1478,8 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
     hipLaunchKernelGGL(k_mark_T<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_mark_G<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 1);
+    if (learn)
+      for (int l = m.maxL - 2; l >= 0; --l) hipLaunchKernelGGL(k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);
   });
 }
 

@@ -40,7 +40,7 @@ void launch_dense_node(hipStream_t s, const SceneDesc* d, const Topo& T, int* ou
 // learned refinement prediction (learned.hip), single scene
 void launch_unit_features(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U);
 void launch_unit_predict(hipStream_t s, const LearnDesc& U, int L, int* cls_out);
-void launch_unit_activate(hipStream_t s, const SceneDesc* d, int dim, const LearnDesc& U);
+void launch_unit_activate(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U);
 void launch_unit_labels_clear(hipStream_t s, const LearnDesc& U);
 void launch_unit_labels_accumulate(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U);
 void launch_unit_label_classes(hipStream_t s, const LearnDesc& U, int L, int* out);

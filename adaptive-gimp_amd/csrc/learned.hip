@@ -100,7 +100,7 @@ __device__ __forceinline__ float op_max(float a, float b) { return fmaxf(a, b); 
 // the units they lie in, for g >= ulev (coarser requests are not predicted)
 template <int D>
 __device__ __forceinline__ void record_requests(const Topo& T, int ulev, const float* x, int g, int* dst,
-                                                int stride) {
+                                                int stride, const int* near = nullptr) {
   if (g < ulev || g >= T.L) return;
   const LevelDesc& Lg = T.lv[g];
   const LevelDesc& Lu = T.lv[ulev];
@@ -113,12 +113,19 @@ __device__ __forceinline__ void record_requests(const Topo& T, int ulev, const f
     int Iu[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) Iu[a] = (b[a] + ((t >> (D - 1 - a)) & 1)) >> (g - ulev);
-    const int64_t q = unit_lidx<D>(Lu, Iu);
+    bool far = false;  // only units within one unit of `near` (the candidates around it)
+    if (near) {
+#pragma unroll
+      for (int a = 0; a < D; ++a) far = far || Iu[a] < near[a] - 1 || Iu[a] > near[a] + 1;
+    }
+    const int64_t q = far ? -1 : unit_lidx<D>(Lu, Iu);
     seen[t] = q;
     bool dup = q < 0;
 #pragma unroll
     for (int s = 0; s < t; ++s) dup = dup || seen[s] == q;
-    if (!dup) atomicMax(&dst[q * stride], g + 1);
+    // most units already hold this level (many particles request the same units): a plain
+    // read first skips the contended atomic; a stale read only costs a redundant atomic
+    if (!dup && dst[q * stride] < g + 1) atomicMax(&dst[q * stride], g + 1);
   }
 }
 
@@ -166,12 +173,13 @@ __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, Le
   float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
   float m = 0.f, vn = 0.f, cn = 0.f, jm1 = 0.f, njac = 0.f;
   if (live) {
-    int I[3];
+    int I[3], Iown[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       x[a] = P.f[(size_t)(PL::X + a) * P.stride + k];
       v[a] = P.f[(size_t)(PL::V + a) * P.stride + k];
       I[a] = (int)floorf(x[a] * Lu.inv_dx);
+      Iown[a] = I[a];
     }
     const int64_t q = unit_lidx<D>(Lu, I);
     u = (int)q;
@@ -198,7 +206,14 @@ __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, Le
     jm1 = J - 1.f;
     njac = -J;
     // request cells lie within one unit of the particle's unit: all are candidates
-    if (u >= 0) record_requests<D>(T, U.ulev, x, g, reinterpret_cast<int*>(U.acc) + UA_REQ, UNIT_ACC);
+    if (u >= 0) {
+      record_requests<D>(T, U.ulev, x, g, reinterpret_cast<int*>(U.acc) + UA_REQ, UNIT_ACC);
+      // the same requests after a straight move over the window (one rounding: fma)
+      float xf[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) xf[a] = __fmaf_rn(v[a], U.win, x[a]);
+      record_requests<D>(T, U.ulev, xf, g, reinterpret_cast<int*>(U.acc) + UA_REQF, UNIT_ACC, Iown);
+    }
   }
   const bool in = live && u >= 0;
   const int key = in ? u : -2 - (int)__lane_id();
@@ -306,7 +321,7 @@ __global__ void k_unit_features(const SceneDesc* __restrict__ Dsc, LearnDesc U) 
     f[4] = af[UA_JSUM] * ic * 10.f;
     f[5] = occ ? (-funkey(au[UA_JMIN]) - 1.f) * 10.f : 0.f;
     f[6] = lev_code((int)au[UA_GMAX], L);
-    f[7] = occ ? lev_code((int)lrintf(af[UA_GSUM] * ic) + 1, L) : 0.f;
+    f[7] = lev_code((int)au[UA_REQF], L);
     f[8] = lev_code((int)au[UA_REQ], L);
     float vb[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -404,6 +419,48 @@ __global__ void __launch_bounds__(PRED_T) k_unit_predict(LearnDesc U, int L, int
   }
 }
 
+// The requests (adaptive_gimp.py:87-96) a particle makes while it moves in a straight line
+// over the window: per axis the cells from min(b0, b1) to max(b0, b1) + 1, with b0 the base
+// cell now and b1 the base cell of x + v * window (at most 3^d cells for moves shorter than a
+// cell).  Part of every predicted octree (`mode` bit 1); the network's unit levels are the
+// other part (`mode` bit 0).
+template <int D>
+__global__ void k_activate_ahead(const SceneDesc* __restrict__ Dsc, int par, LearnDesc U) {
+  const SceneDesc& sd = Dsc[0];
+  const Topo& T = sd.T;
+  const PBuf& P = sd.P[par];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= sd.n) return;
+  const int g = P.g[k];
+  if (g < 0 || g >= T.L) return;
+  const LevelDesc& Lg = T.lv[g];
+  int lo[3], ext[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const float x = P.f[(size_t)a * P.stride + k];
+    const float xf = __fmaf_rn(P.f[(size_t)(D + a) * P.stride + k], U.win, x);
+    const int b0 = base_cell(x, Lg, T.pow2), b1 = base_cell(xf, Lg, T.pow2);
+    lo[a] = min(b0, b1);
+    ext[a] = min(max(b0, b1) - lo[a], 1) + 2;  // 2 or 3 cells (longer moves are left to the check)
+  }
+  constexpr int NMAX = (D == 2) ? 9 : 27;
+  for (int t = 0; t < NMAX; ++t) {
+    int J[3], r = t;
+    bool in = true;
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) {
+      const int o = r % 3;
+      r /= 3;
+      in = in && o < ext[a];
+      J[a] = lo[a] + o + PAD;
+    }
+    if (!in || !in_box<D>(Lg.E, J)) continue;
+    const uint32_t q = bidx<D>(Lg.NB, J);
+    Lg.cell[q] = ACTIVATED;
+    T.cflag[Lg.gbase + (int)(q / P4<D>::v)] = 1;
+  }
+}
+
 // cells of the predicted level inside every refined unit get ACTIVATED (between the
 // particle activation and the culling of the rebuild)
 template <int D>
@@ -464,8 +521,11 @@ void launch_unit_predict(hipStream_t s, const LearnDesc& U, int L, int* cls_out)
   hipLaunchKernelGGL(k_unit_predict, dim3(512), dim3(PRED_T), 0, s, U, L, cls_out);
 }
 
-void launch_unit_activate(hipStream_t s, const SceneDesc* d, int dim, const LearnDesc& U) {
-  LEARN_DIM(dim, hipLaunchKernelGGL(k_unit_activate<DD>, dim3(2048), dim3(256), 0, s, d, U));
+void launch_unit_activate(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U) {
+  LEARN_DIM(dim, {
+    if (U.mode & 1) hipLaunchKernelGGL(k_unit_activate<DD>, dim3(2048), dim3(256), 0, s, d, U);
+    if (U.mode & 2) hipLaunchKernelGGL(k_activate_ahead<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U);
+  });
 }
 
 void launch_unit_labels_clear(hipStream_t s, const LearnDesc& U) {

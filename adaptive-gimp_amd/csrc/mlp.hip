@@ -59,7 +59,8 @@ __device__ __forceinline__ void wave_sync_lds() {
 //   loss = scale * sum_rows cw[y] * (-log softmax(z)[y]),  grad likewise.
 // grad [5508] and loss [1] are accumulated with atomics (zeroed by the caller).
 __global__ void __launch_bounds__(MLP_T) k_mlp_grad(const float* __restrict__ X, const int* __restrict__ Y,
-                                                    int64_t rows, const float* __restrict__ params,
+                                                    const int* __restrict__ idx, int64_t rows,
+                                                    const float* __restrict__ params,
                                                     const float* __restrict__ cw, float scale,
                                                     float* __restrict__ grad, float* __restrict__ loss) {
   __shared__ __align__(16) MlpLds W;
@@ -87,12 +88,12 @@ __global__ void __launch_bounds__(MLP_T) k_mlp_grad(const float* __restrict__ X,
   }
   for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
     MlpTile t;
-    mlp_forward_tile(W, X, rows, tile * 16, t);
+    mlp_forward_tile(W, X, rows, tile * 16, t, idx);
     const int64_t row = tile * 16 + n;
     // dz = scale * cw[y] * (softmax(z) - onehot(y)); lanes 0..15 own one row each
     float dz[4] = {0.f, 0.f, 0.f, 0.f};
     if (g == 0 && row < rows) {
-      const int y = Y[row];
+      const int y = Y[idx ? (int64_t)idx[row] : row];
       const float w = (cw ? cw[y] : 1.f) * scale;
       float mx = fmaxf(fmaxf(t.z[0], t.z[1]), fmaxf(t.z[2], t.z[3]));
       float e[4], se = 0.f;
@@ -249,9 +250,43 @@ static int mlp_grid(int64_t rows) {
   return (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 1024));
 }
 
+static void launch_grad(const float* X, const int* y, const int* idx, int64_t rows, const float* params,
+                        const float* cw, float scale, float* grad, float* loss, bool zero_loss, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_f32, dim3(8), dim3(256), 0, s, grad, 0.f, (int64_t)MLP_NPARAM);
+  if (zero_loss) hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, loss, 0.f, (int64_t)1);
+  if (rows > 0)
+    hipLaunchKernelGGL(k_mlp_grad, dim3(std::min(mlp_grid(rows), 128)), dim3(MLP_T), 0, s, X, y, idx, rows, params,
+                       cw, scale, grad, loss);
+}
+
+static void launch_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr, float beta1,
+                        float beta2, float eps, float wd, hipStream_t s) {
+  const float c1 = 1.f - std::pow(beta1, (float)step), c2 = 1.f - std::pow(beta2, (float)step);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, s, params, grad, m, v, lr, beta1,
+                     beta2, eps, c1, c2, wd);
+}
+
 }  // namespace agimp
 
 using namespace agimp;
+
+// Device-resident training state: dataset, parameters, Adam moments, scratch.
+struct agimp_trainer {
+  int device = 0;
+  hipStream_t s = nullptr;
+  int64_t cap = 0, rows = 0;
+  float* X = nullptr;     // [cap][16]
+  int* y = nullptr;       // [cap]
+  int* perm = nullptr;    // [cap]
+  float* Z = nullptr;     // [cap][4]
+  float* p = nullptr;     // [5508] parameters, then m, v, grad (one block)
+  float* m = nullptr;
+  float* v = nullptr;
+  float* g = nullptr;
+  float* loss = nullptr;  // [1]
+  float* cw = nullptr;    // [4]
+  int32_t step = 0;
+};
 
 extern "C" {
 
@@ -266,22 +301,151 @@ int agimp_mlp_forward(const float* X, int64_t rows, const float* params, float* 
 int agimp_mlp_loss_grad(const float* X, const int32_t* y, int64_t rows, const float* params,
                         const float* class_weight, float scale, float* grad, float* loss, void* stream) {
   if (!X || !y || !params || !grad || !loss || rows < 0) return AGIMP_EINVAL;
-  hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_fill_f32, dim3(8), dim3(256), 0, s, grad, 0.f, (int64_t)MLP_NPARAM);
-  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, loss, 0.f, (int64_t)1);
-  if (rows > 0)
-    hipLaunchKernelGGL(k_mlp_grad, dim3(std::min(mlp_grid(rows), 128)), dim3(MLP_T), 0, s, X, y, rows, params,
-                       class_weight, scale, grad, loss);
+  launch_grad(X, y, nullptr, rows, params, class_weight, scale, grad, loss, true, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? 0 : AGIMP_EHIP;
 }
 
 int agimp_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr, float beta1,
                    float beta2, float eps, float weight_decay, void* stream) {
   if (!params || !grad || !m || !v || step < 1) return AGIMP_EINVAL;
-  const float c1 = 1.f - std::pow(beta1, (float)step), c2 = 1.f - std::pow(beta2, (float)step);
-  hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, (hipStream_t)stream, params, grad, m, v,
-                     lr, beta1, beta2, eps, c1, c2, weight_decay);
+  launch_adam(params, grad, m, v, step, lr, beta1, beta2, eps, weight_decay, (hipStream_t)stream);
   return hipGetLastError() == hipSuccess ? 0 : AGIMP_EHIP;
+}
+
+int agimp_trainer_destroy(agimp_trainer* t) {
+  if (!t) return AGIMP_EINVAL;
+  if (t->s) (void)hipStreamSynchronize(t->s);
+  for (void* q : {(void*)t->X, (void*)t->y, (void*)t->perm, (void*)t->Z, (void*)t->p})
+    if (q) (void)hipFree(q);
+  if (t->s) (void)hipStreamDestroy(t->s);
+  delete t;
+  return 0;
+}
+
+int agimp_trainer_create(int64_t capacity, int32_t device, agimp_trainer** out) {
+  if (!out || capacity < 1) return AGIMP_EINVAL;
+  *out = nullptr;
+  agimp_trainer* t = new agimp_trainer();
+  t->device = device;
+  t->cap = capacity;
+  bool ok = hipSetDevice(device) == hipSuccess && hipStreamCreateWithFlags(&t->s, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&t->X, sizeof(float) * MLP_IN * capacity) == hipSuccess &&
+            hipMalloc(&t->y, sizeof(int) * capacity) == hipSuccess &&
+            hipMalloc(&t->perm, sizeof(int) * capacity) == hipSuccess &&
+            hipMalloc(&t->Z, sizeof(float) * MLP_OUT * capacity) == hipSuccess &&
+            hipMalloc(&t->p, sizeof(float) * (4 * MLP_NPARAM + 8)) == hipSuccess;
+  if (!ok) {
+    agimp_trainer_destroy(t);
+    return AGIMP_ENOMEM;
+  }
+  t->m = t->p + MLP_NPARAM;
+  t->v = t->m + MLP_NPARAM;
+  t->g = t->v + MLP_NPARAM;
+  t->loss = t->g + MLP_NPARAM;
+  t->cw = t->loss + 4;
+  const float ones[4] = {1.f, 1.f, 1.f, 1.f};
+  if (hipMemsetAsync(t->p, 0, sizeof(float) * (4 * MLP_NPARAM + 8), t->s) != hipSuccess ||
+      hipMemcpyAsync(t->cw, ones, sizeof(ones), hipMemcpyHostToDevice, t->s) != hipSuccess ||
+      hipStreamSynchronize(t->s) != hipSuccess) {
+    agimp_trainer_destroy(t);
+    return AGIMP_EHIP;
+  }
+  *out = t;
+  return 0;
+}
+
+int agimp_trainer_set_data(agimp_trainer* t, const float* X, const int32_t* y, int64_t rows) {
+  if (!t || !X || !y || rows < 0 || rows > t->cap) return AGIMP_EINVAL;
+  for (int64_t i = 0; i < rows; ++i)
+    if (y[i] < 0 || y[i] >= MLP_OUT) return AGIMP_EINVAL;
+  if (hipMemcpyAsync(t->X, X, sizeof(float) * MLP_IN * rows, hipMemcpyHostToDevice, t->s) != hipSuccess ||
+      hipMemcpyAsync(t->y, y, sizeof(int) * rows, hipMemcpyHostToDevice, t->s) != hipSuccess ||
+      hipStreamSynchronize(t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  t->rows = rows;
+  return 0;
+}
+
+int agimp_trainer_set_params(agimp_trainer* t, const float* params) {
+  if (!t || !params) return AGIMP_EINVAL;
+  if (hipMemcpyAsync(t->p, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, t->s) != hipSuccess ||
+      hipMemsetAsync(t->m, 0, sizeof(float) * 2 * MLP_NPARAM, t->s) != hipSuccess ||
+      hipStreamSynchronize(t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  t->step = 0;
+  return 0;
+}
+
+int agimp_trainer_get_params(agimp_trainer* t, float* params) {
+  if (!t || !params) return AGIMP_EINVAL;
+  if (hipMemcpyAsync(params, t->p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, t->s) != hipSuccess ||
+      hipStreamSynchronize(t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  return 0;
+}
+
+int agimp_trainer_epoch(agimp_trainer* t, const int32_t* perm, int64_t batch, const float* class_weight, float scale,
+                        float lr, float weight_decay, double* mean_loss) {
+  if (!t || !perm || batch < 1 || !class_weight) return AGIMP_EINVAL;
+  const int64_t rows = t->rows;
+  for (int64_t i = 0; i < rows; ++i)
+    if (perm[i] < 0 || perm[i] >= rows) return AGIMP_EINVAL;
+  if (hipMemcpyAsync(t->perm, perm, sizeof(int) * rows, hipMemcpyHostToDevice, t->s) != hipSuccess ||
+      hipMemcpyAsync(t->cw, class_weight, sizeof(float) * 4, hipMemcpyHostToDevice, t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, t->s, t->loss, 0.f, (int64_t)1);
+  int64_t nb = 0;
+  for (int64_t b0 = 0; b0 + batch <= rows || (b0 == 0 && rows > 0); b0 += batch) {
+    const int64_t bs = std::min(batch, rows - b0);
+    launch_grad(t->X, t->y, t->perm + b0, bs, t->p, t->cw, scale, t->g, t->loss, false, t->s);
+    launch_adam(t->p, t->g, t->m, t->v, ++t->step, lr, 0.9f, 0.999f, 1e-8f, weight_decay, t->s);
+    ++nb;
+  }
+  float l = 0.f;
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(&l, t->loss, sizeof(float), hipMemcpyDeviceToHost, t->s) != hipSuccess ||
+      hipStreamSynchronize(t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  if (mean_loss) *mean_loss = nb ? (double)l / (double)nb : 0.0;
+  return 0;
+}
+
+int agimp_trainer_logits(agimp_trainer* t, const float* X, int64_t rows, float* Z) {
+  if (!t || !X || !Z || rows < 0 || rows > t->cap) return AGIMP_EINVAL;
+  if (rows == 0) return 0;
+  if (hipMemcpyAsync(t->X, X, sizeof(float) * MLP_IN * rows, hipMemcpyHostToDevice, t->s) != hipSuccess) return AGIMP_EHIP;
+  t->rows = 0;  // the dataset was overwritten
+  hipLaunchKernelGGL(k_mlp_forward, dim3(mlp_grid(rows)), dim3(MLP_T), 0, t->s, t->X, rows, t->p, t->Z);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(Z, t->Z, sizeof(float) * MLP_OUT * rows, hipMemcpyDeviceToHost, t->s) != hipSuccess ||
+      hipStreamSynchronize(t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  return 0;
+}
+
+int agimp_trainer_grad(agimp_trainer* t, const float* X, const int32_t* y, int64_t rows, const float* class_weight,
+                       float scale, float* grad, float* loss) {
+  if (!t || !X || !y || !grad || !loss || rows < 0 || rows > t->cap) return AGIMP_EINVAL;
+  int rc = agimp_trainer_set_data(t, X, y, rows);
+  if (rc) return rc;
+  if (class_weight && hipMemcpyAsync(t->cw, class_weight, sizeof(float) * 4, hipMemcpyHostToDevice, t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  launch_grad(t->X, t->y, nullptr, rows, t->p, class_weight ? t->cw : nullptr, scale, t->g, t->loss, true, t->s);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(grad, t->g, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost, t->s) != hipSuccess ||
+      hipMemcpyAsync(loss, t->loss, sizeof(float), hipMemcpyDeviceToHost, t->s) != hipSuccess ||
+      hipStreamSynchronize(t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  return 0;
+}
+
+int agimp_trainer_apply(agimp_trainer* t, const float* grad, float lr, float weight_decay) {
+  if (!t || !grad) return AGIMP_EINVAL;
+  if (hipMemcpyAsync(t->g, grad, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice, t->s) != hipSuccess)
+    return AGIMP_EHIP;
+  launch_adam(t->p, t->g, t->m, t->v, ++t->step, lr, 0.9f, 0.999f, 1e-8f, weight_decay, t->s);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(t->s) != hipSuccess) return AGIMP_EHIP;
+  return 0;
 }
 
 }  // extern "C"

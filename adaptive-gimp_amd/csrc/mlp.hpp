@@ -94,14 +94,17 @@ struct MlpTile {
   f32x4 z;             // logits: lanes 0..15 hold z[row = lane][j]
 };
 
-// Forward pass of rows r0 .. r0+15 of X [rows][16] (rows past `rows` read as 0).
+// Forward pass of rows r0 .. r0+15 (rows past `rows` read as 0); row r is X[idx ? idx[r] : r].
 __device__ __forceinline__ void mlp_forward_tile(const MlpLds& W, const float* __restrict__ X, int64_t rows,
-                                                 int64_t r0, MlpTile& t) {
+                                                 int64_t r0, MlpTile& t, const int* __restrict__ idx = nullptr) {
   const int lane = __lane_id();
   const int n = lane & 15, g = lane >> 4;
   const int64_t row = r0 + n;
   float4 xv = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (row < rows) xv = *reinterpret_cast<const float4*>(X + row * MLP_IN + 4 * g);
+  if (row < rows) {
+    const int64_t src = idx ? (int64_t)idx[row] : row;
+    xv = *reinterpret_cast<const float4*>(X + src * MLP_IN + 4 * g);
+  }
   t.xb = pack4(xv.x, xv.y, xv.z, xv.w);
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

@@ -9,8 +9,9 @@ Not in the reference, which rebuilds its octree from the current particles every
 * its label is the class of the finest level the plain criterion requests inside the unit
   during substeps t .. t+window-1 of a plain run (0 finest, 1 one coarser, 2 two coarser,
   3 none or coarser);
-* the network (16-64-64-4) is trained by softmax cross-entropy with Adam; forward, backward
-  and the update all run in the library's kernels, PyTorch only holds the device tensors.
+* the network (16-64-64-4) is trained by class-weighted softmax cross-entropy with Adam;
+  the dataset, parameters and optimiser state live in device memory owned by the library
+  (agimp_trainer_*), and forward, backward and update run in its kernels.
 
     python adaptive-gimp_amd/learned.py train --runs 200 --out refine_mlp.npz
 
@@ -107,55 +108,89 @@ def generate(seeds, n=200_000, windows=10, window=WINDOW, log=None):
 
 # ----------------------------------------------------------------------------- training
 class Trainer:
-    """Adam on the library's kernels; tensors live on the GPU (PyTorch as device memory)."""
+    """The library's device-resident trainer (agimp_trainer_*): forward, backward and Adam
+    run in the library's kernels on data it holds in device memory."""
 
-    def __init__(self, params, lr=3e-3, weight_decay=1e-4):
-        import torch
+    def __init__(self, capacity, params=None, device=0):
+        self._lib = _pkg()._lib
+        self.L = self._lib.load()
+        h = ctypes.c_void_p()
+        self._lib.check(self.L.agimp_trainer_create(int(capacity), int(device), ctypes.byref(h)),
+                        what="agimp_trainer_create")
+        self.h = h
+        self.cap = int(capacity)
+        self.set_params(init_params(0) if params is None else params)
 
-        self.torch = torch
-        self.L = _pkg()._lib.load()
-        dev = torch.device("cuda")
-        self.p = torch.tensor(np.asarray(params, np.float32), device=dev)
-        self.m = torch.zeros_like(self.p)
-        self.v = torch.zeros_like(self.p)
-        self.g = torch.zeros_like(self.p)
-        self.loss = torch.zeros(1, device=dev)
-        self.step_no = 0
-        self.lr, self.wd = lr, weight_decay
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.agimp_trainer_destroy(self.h)
+            self.h = None
 
-    def _s(self):
-        return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
-    def grad(self, X, y, cw=None, scale=None):
-        """loss and gradient (device tensors) of scale * sum cw[y] CE on the rows (X, y)."""
-        rows = X.shape[0]
-        if scale is None:
-            scale = 1.0 / float(cw[y].sum().item()) if cw is not None else 1.0 / rows
-        rc = self.L.agimp_mlp_loss_grad(ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(y.data_ptr()), rows,
-                                        ctypes.c_void_p(self.p.data_ptr()),
-                                        ctypes.c_void_p(cw.data_ptr()) if cw is not None else None,
-                                        float(scale), ctypes.c_void_p(self.g.data_ptr()),
-                                        ctypes.c_void_p(self.loss.data_ptr()), self._s())
-        if rc != 0:
-            raise RuntimeError(f"agimp_mlp_loss_grad failed ({rc})")
-        return self.loss, self.g
+    def _f(self, a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
-    def step(self, lr=None):
-        self.step_no += 1
-        rc = self.L.agimp_mlp_adam(ctypes.c_void_p(self.p.data_ptr()), ctypes.c_void_p(self.g.data_ptr()),
-                                   ctypes.c_void_p(self.m.data_ptr()), ctypes.c_void_p(self.v.data_ptr()),
-                                   self.step_no, float(lr if lr is not None else self.lr), 0.9, 0.999, 1e-8,
-                                   float(self.wd), self._s())
-        if rc != 0:
-            raise RuntimeError(f"agimp_mlp_adam failed ({rc})")
+    def _i(self, a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        self._lib.check(self.L.agimp_trainer_set_params(self.h, self._f(p)), what="agimp_trainer_set_params")
+
+    def params(self):
+        p = np.empty(NPARAM, np.float32)
+        self._lib.check(self.L.agimp_trainer_get_params(self.h, self._f(p)), what="agimp_trainer_get_params")
+        return p
+
+    def set_data(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        self._lib.check(self.L.agimp_trainer_set_data(self.h, self._f(X), self._i(y), len(y)),
+                        what="agimp_trainer_set_data")
+        self.rows = len(y)
+
+    def epoch(self, perm, batch, class_weight, lr, weight_decay=1e-4):
+        perm = np.ascontiguousarray(perm, np.int32)
+        cw = np.ascontiguousarray(class_weight, np.float32)
+        out = ctypes.c_double(0.0)
+        self._lib.check(self.L.agimp_trainer_epoch(self.h, self._i(perm), int(batch), self._f(cw),
+                                                   1.0 / min(batch, self.rows), float(lr), float(weight_decay),
+                                                   ctypes.byref(out)), what="agimp_trainer_epoch")
+        return out.value
 
     def logits(self, X):
-        out = self.torch.empty((X.shape[0], 4), device=X.device, dtype=self.torch.float32)
-        rc = self.L.agimp_mlp_forward(ctypes.c_void_p(X.data_ptr()), X.shape[0], ctypes.c_void_p(self.p.data_ptr()),
-                                      ctypes.c_void_p(out.data_ptr()), self._s())
-        if rc != 0:
-            raise RuntimeError(f"agimp_mlp_forward failed ({rc})")
-        return out
+        X = np.ascontiguousarray(X, np.float32)
+        Z = np.empty((len(X), 4), np.float32)
+        for a in range(0, len(X), self.cap):
+            b = min(len(X), a + self.cap)
+            self._lib.check(self.L.agimp_trainer_logits(self.h, self._f(X[a:b]), b - a, self._f(Z[a:b])),
+                            what="agimp_trainer_logits")
+        self.rows = 0
+        return Z
+
+    def grad(self, X, y, class_weight=None, scale=None):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.int32)
+        cw = None if class_weight is None else np.ascontiguousarray(class_weight, np.float32)
+        if scale is None:
+            scale = 1.0 / (float(cw[y].sum()) if cw is not None else len(y))
+        g = np.empty(NPARAM, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._lib.check(self.L.agimp_trainer_grad(self.h, self._f(X), self._i(y), len(y),
+                                                  self._f(cw) if cw is not None else None, float(scale),
+                                                  self._f(g), self._f(loss)), what="agimp_trainer_grad")
+        self.rows = 0
+        return float(loss[0]), g
+
+    def apply(self, g, lr, weight_decay=0.0):
+        g = np.ascontiguousarray(g, np.float32)
+        self._lib.check(self.L.agimp_trainer_apply(self.h, self._f(g), float(lr), float(weight_decay)),
+                        what="agimp_trainer_apply")
 
 
 def decide(z, tau):
@@ -170,49 +205,35 @@ def decide(z, tau):
     return c
 
 
-def train(X, y, X_val=None, y_val=None, epochs=8, batch=16384, lr=3e-3, seed=0, log=None):
-    import torch
-
-    tr = Trainer(init_params(seed), lr=lr)
-    dev = torch.device("cuda")
-    Xd = torch.tensor(np.ascontiguousarray(X, np.float32), device=dev)
-    yd = torch.tensor(np.ascontiguousarray(y, np.int32), device=dev)
+def class_weights(y):
+    """Inverse-square-root class frequencies, normalised to a mean of 1 over the rows."""
     freq = np.bincount(y, minlength=4).astype(np.float64) + 1.0
-    cw_np = (freq.sum() / freq) ** 0.5
-    cw_np /= cw_np[y].mean() if len(y) else 1.0
-    cw = torch.tensor(cw_np.astype(np.float32), device=dev)
-    n = len(y)
-    steps = epochs * max(1, n // batch)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(seed)
+    cw = (freq.sum() / freq) ** 0.5
+    return cw / cw[y].mean()
+
+
+def train(X, y, X_val=None, y_val=None, epochs=8, batch=16384, lr=3e-3, seed=0, log=None):
+    rng = np.random.default_rng(seed)
+    tr = Trainer(len(y), init_params(seed))
+    cw = class_weights(y)
     hist = []
-    k = 0
     for ep in range(epochs):
-        perm = torch.randperm(n, device=dev, generator=gen)
-        tot, cnt = 0.0, 0
-        for b0 in range(0, n - batch + 1 if n >= batch else 1, batch):
-            idx = perm[b0:b0 + batch]
-            xb = Xd.index_select(0, idx).contiguous()
-            yb = yd.index_select(0, idx).contiguous()
-            loss, _ = tr.grad(xb, yb, cw)
-            cur = lr * 0.5 * (1 + np.cos(np.pi * k / steps))  # cosine decay
-            tr.step(max(cur, lr * 0.02))
-            k += 1
-            tot += float(loss.item())
-            cnt += 1
-        rec = {"epoch": ep, "loss": tot / max(cnt, 1)}
+        tr.set_data(X, y)
+        cur = lr * (0.02 + 0.98 * 0.5 * (1 + np.cos(np.pi * ep / max(1, epochs - 1))))  # cosine decay per epoch
+        loss = tr.epoch(rng.permutation(len(y)), batch, cw, cur)
+        rec = {"epoch": ep, "lr": float(cur), "loss": float(loss)}
         if X_val is not None:
             rec.update(evaluate(tr, X_val, y_val))
         hist.append(rec)
         if log:
             log(json.dumps(rec))
-    return tr.p.cpu().numpy(), hist, cw_np
+    p = tr.params()
+    tr.close()
+    return p, hist, cw
 
 
-def evaluate(tr, X, y, taus=(0.2, 0.3, 0.5)):
-    import torch
-
-    z = tr.logits(torch.tensor(np.ascontiguousarray(X, np.float32), device="cuda")).cpu().numpy()
+def evaluate(tr, X, y, taus=(0.1, 0.2, 0.3, 0.5)):
+    z = tr.logits(X)
     out = {"val_acc_argmax": float(np.mean(np.argmax(z, 1) == y))}
     for t in taus:
         c = decide(z, t)

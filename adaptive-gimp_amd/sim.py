@@ -348,6 +348,12 @@ class AdaptiveGIMP:
         _lib.check(self._L.agimp_learned_advance(self._h, float(dt0), int(n)), self._h, "agimp_learned_advance")
         self._invalidate()
 
+    def learned_set_parts(self, network=True, sweep=True):
+        """Parts of a predicted octree besides the current requests: the network's unit levels
+        and/or each particle's requests swept along its velocity over the window."""
+        _lib.check(self._L.agimp_learned_set_parts(self._h, int(bool(network)) | (2 * int(bool(sweep)))), self._h,
+                   "agimp_learned_set_parts")
+
     def learned_stats(self):
         out = np.zeros(len(_lib.LSTATS), np.int64)
         _lib.check(self._L.agimp_learned_stats(self._h, _lib.ptr(out, ctypes.c_int64), len(out)), self._h,

@@ -156,8 +156,10 @@ const char* agimp_batch_last_error(agimp_batch* b);
  * and levels, mean displacement over the window, upwind level).  A 16-64-64-4 network
  * (agimp_mlp_*) classifies each candidate by the finest level the plain criterion will
  * request in it during the next `window` substeps: 0 finest, 1 one coarser, 2 two coarser,
- * 3 none.  The predicted octree is the plain one of the current particles plus, in every
- * unit of class c < 3, all cells of level (level-1-c) if that is not coarser than the unit.
+ * 3 none.  The predicted octree is the plain one of the current particles plus two parts
+ * (agimp_learned_set_parts): the network's (in every unit of class c < 3, all cells of level
+ * level-1-c if that is not coarser than the unit) and the swept requests (each particle's
+ * request cells at g_p between its base cell now and the base cell of x + v * window dt).
  *
  * agimp_learned_advance reuses one predicted octree for up to `window` substeps (captured
  * bucketing + physics replays).  Every substep checks the plain criterion in the level
@@ -196,6 +198,10 @@ int agimp_learned_labels(agimp_handle* h, int32_t* cls, int64_t cap);
 int agimp_learned_predict(agimp_handle* h, float dt, int32_t* cls, int8_t* level, int64_t cap, int64_t* count);
 /* n substeps with predicted octrees (see above). */
 int agimp_learned_advance(agimp_handle* h, float dt, int32_t n);
+#define AGIMP_PREDICT_NETWORK 1
+#define AGIMP_PREDICT_SWEEP 2
+/* Which parts a predicted octree holds besides the current requests (default both). */
+int agimp_learned_set_parts(agimp_handle* h, int32_t parts);
 int agimp_learned_stats(agimp_handle* h, int64_t* out, int32_t n);
 int agimp_learned_reset_stats(agimp_handle* h);
 
@@ -211,6 +217,30 @@ int agimp_mlp_loss_grad(const float* X, const int32_t* y, int64_t rows, const fl
 /* One Adam step (bias-corrected; decoupled weight decay on the weight matrices). */
 int agimp_mlp_adam(float* params, const float* grad, float* m, float* v, int32_t step, float lr, float beta1,
                    float beta2, float eps, float weight_decay, void* stream);
+
+/* Device-resident training of the network (no other GPU library needed): the trainer owns
+ * the dataset (capacity rows of 16 features and a class 0..3), the parameters, the Adam
+ * moments and scratch, all on `device`.  Host arrays are borrowed for the call. */
+typedef struct agimp_trainer agimp_trainer;
+int agimp_trainer_create(int64_t capacity, int32_t device, agimp_trainer** out);
+int agimp_trainer_destroy(agimp_trainer* t);
+int agimp_trainer_set_data(agimp_trainer* t, const float* X, const int32_t* y, int64_t rows);
+/* set the parameters and reset the Adam state */
+int agimp_trainer_set_params(agimp_trainer* t, const float* params);
+int agimp_trainer_get_params(agimp_trainer* t, float* params);
+/* One pass over the dataset in the order perm[0..rows) (a permutation of the rows): for every
+ * full batch (and a single short one when rows < batch), the gradient of
+ * scale * sum cw[y] CE over the batch and one Adam step (beta 0.9/0.999, eps 1e-8).
+ * mean_loss receives the mean of the batch losses. */
+int agimp_trainer_epoch(agimp_trainer* t, const int32_t* perm, int64_t batch, const float* class_weight, float scale,
+                        float lr, float weight_decay, double* mean_loss);
+/* Logits [rows][4] of host rows X [rows][16] (overwrites the dataset). */
+int agimp_trainer_logits(agimp_trainer* t, const float* X, int64_t rows, float* Z);
+/* Loss and gradient [5508] of host rows (overwrites the dataset); class_weight may be NULL. */
+int agimp_trainer_grad(agimp_trainer* t, const float* X, const int32_t* y, int64_t rows, const float* class_weight,
+                       float scale, float* grad, float* loss);
+/* One Adam step with a given gradient (host). */
+int agimp_trainer_apply(agimp_trainer* t, const float* grad, float lr, float weight_decay);
 
 /* Host-side FP64 maths used by the kernels, exported for CPU checks: the rotation-variant
  * SVD (dim 2 or 3, row-major) and the constitutive update (clamp plasticity if enabled in

@@ -89,8 +89,9 @@ class Units:
         q[ok] = bidx(J[ok], self.NB, self.d)
         return q
 
-    def requests(self, x, g, only=None):
-        """Max (level + 1) requested per unit (dense [nu] int), levels >= ulev."""
+    def requests(self, x, g, only=None, near=None):
+        """Max (level + 1) requested per unit (dense [nu] int), levels >= ulev; with `near`
+        (unit coordinates per particle) only units within one unit of it are recorded."""
         req = np.zeros(self.nu, np.int64)
         sel = (g >= self.ulev) & (g < self.L)
         if only is not None:
@@ -104,8 +105,21 @@ class Units:
                 Iu = (b + np.asarray(dl)) >> (lev - self.ulev)
                 J = Iu + PAD
                 ok = np.all((J >= 0) & (J < self.E), axis=1)
+                if near is not None:
+                    ok &= np.all(np.abs(Iu - near[s]) <= 1, axis=1)
                 np.maximum.at(req, bidx(J[ok], self.NB, self.d), lev + 1)
         return req
+
+    def unit_coords_of(self, x):
+        return np.floor(np.asarray(x, np.float32) * np.float32(self.c << self.ulev)).astype(np.int64)
+
+    @staticmethod
+    def fma32(a, b, c):
+        """float32 fma(a, b, c) with a single rounding (extended precision holds the exact
+        value for the magnitudes here)."""
+        r = np.asarray(a, np.float32).astype(np.longdouble) * np.asarray(b, np.float32).astype(np.longdouble) \
+            + np.asarray(c, np.float32).astype(np.longdouble)
+        return r.astype(np.float32)
 
     def candidates(self, occupied):
         """Units within one unit (3^d neighbourhood, in the box) of an occupied unit."""
@@ -143,6 +157,9 @@ class Units:
         gmax = np.zeros(nu, np.int64)
         np.maximum.at(gmax, q[ok], g[ok] + 1)
         req = self.requests(x, g, only=ok)
+        win32 = np.float32(np.float32(window) * np.float32(dt))
+        xf = self.fma32(v, win32, x)
+        reqf = self.requests(xf, g, only=ok, near=self.unit_coords_of(x))
         occ = cnt > 0
         cand = self.candidates(occ)
         dxu = np.float64(level_dx(self.c, self.ulev))
@@ -163,7 +180,7 @@ class Units:
             f[4] = jsum[u] * ic * 10
             f[5] = (jmin[u] - 1) * 10 if o else 0.0
             f[6] = lev_code(gmax[u], L)
-            f[7] = lev_code(int(np.rint(gsum[u] * ic)) + 1, L) if o else 0.0
+            f[7] = lev_code(reqf[u], L)
             f[8] = lev_code(req[u], L)
             f[12:12 + d] = vb[u] * ic * disp
             nm, ng, nr, up = 0.0, 0.0, 0.0, 0.0

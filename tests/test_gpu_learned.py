@@ -6,20 +6,10 @@ Bars (stated per test): the network's logits and gradients follow the oracle's B
 points, so they agree to FP32 accumulation order (plus rare one-ulp BF16 rounding flips
 where FP32 and FP64 intermediates straddle a rounding boundary); features are FP32 sums.
 """
-import ctypes
-
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
-
-
-@pytest.fixture(scope="module")
-def torch():
-    t = pytest.importorskip("torch")
-    if not t.cuda.is_available():
-        pytest.skip("no GPU")
-    return t
 
 
 @pytest.fixture(scope="module")
@@ -36,12 +26,9 @@ def UO():
     return units_oracle
 
 
-def _vp(t):
-    return ctypes.c_void_p(t.data_ptr())
-
-
-def _stream(torch):
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+@pytest.fixture(scope="module")
+def learned(pkg):
+    return __import__(pkg.__name__ + ".learned", fromlist=["Trainer"])
 
 
 def _params(M, seed):
@@ -53,23 +40,24 @@ def _params(M, seed):
     return p.astype(np.float32)
 
 
-def test_mlp_forward_matches_oracle(pkg, torch, M):
-    L = pkg._lib.load()
+def test_mlp_forward_matches_oracle(learned, M):
+    """Logits of the BF16 matrix-core forward vs the oracle with the same rounding points:
+    bound 5e-3 of the largest logit (FP32 vs FP64 accumulation, rare BF16 rounding flips)."""
     p = _params(M, 1)
     rng = np.random.default_rng(2)
     X = rng.normal(0, 1, (1007, 16)).astype(np.float32)  # not a multiple of the 16-row tile
-    Xd, pd = torch.tensor(X, device="cuda"), torch.tensor(p, device="cuda")
-    Z = torch.empty((1007, 4), device="cuda")
-    assert L.agimp_mlp_forward(_vp(Xd), 1007, _vp(pd), _vp(Z), _stream(torch)) == 0
-    torch.cuda.synchronize()
+    tr = learned.Trainer(2048, p)
+    Z = tr.logits(X)
+    tr.close()
     ref, _ = M.forward(p, X, bf16_rounding=True)
-    err = np.abs(Z.cpu().numpy() - ref).max() / np.abs(ref).max()
+    err = np.abs(Z - ref).max() / np.abs(ref).max()
     print(f"forward max rel err {err:.2e}")
     assert err < 5e-3
 
 
-def test_mlp_gradient_matches_oracle(pkg, torch, M):
-    L = pkg._lib.load()
+def test_mlp_gradient_matches_oracle(learned, M):
+    """Loss within 1e-5 relative, gradient within 5e-3 relative L2 (1e-2 per parameter block)
+    of the CPU oracle computed in FP64 with the same BF16 rounding points."""
     p = _params(M, 3)
     rng = np.random.default_rng(4)
     rows = 4099
@@ -77,66 +65,56 @@ def test_mlp_gradient_matches_oracle(pkg, torch, M):
     y = rng.integers(0, 4, rows).astype(np.int32)
     cw = np.array([1.0, 2.0, 0.5, 1.5], np.float32)
     scale = 1.0 / float(cw[y].sum())
-    Xd, yd, pd, cwd = (torch.tensor(a, device="cuda") for a in (X, y, p, cw))
-    g = torch.empty(5508, device="cuda")
-    loss = torch.empty(1, device="cuda")
-    assert L.agimp_mlp_loss_grad(_vp(Xd), _vp(yd), rows, _vp(pd), _vp(cwd), scale, _vp(g), _vp(loss),
-                                 _stream(torch)) == 0
-    torch.cuda.synchronize()
+    tr = learned.Trainer(rows, p)
+    loss, gg = tr.grad(X, y, cw, scale)
+    tr.close()
     lref, gref = M.loss_grad(p, X, y, cw, scale, bf16_rounding=True)
-    gg = g.cpu().numpy().astype(np.float64)
+    gg = gg.astype(np.float64)
     rel = np.linalg.norm(gg - gref) / np.linalg.norm(gref)
-    print(f"loss {loss.item():.6f} vs {lref:.6f}; gradient rel L2 err {rel:.2e}")
-    assert abs(loss.item() - lref) <= 1e-5 * abs(lref)
+    print(f"loss {loss:.6f} vs {lref:.6f}; gradient rel L2 err {rel:.2e}")
+    assert abs(loss - lref) <= 1e-5 * abs(lref)
     assert rel < 5e-3
     for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2), (M.OFF_B2, M.OFF_W3),
                  (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.NPARAM)):
         assert np.linalg.norm(gg[a:b] - gref[a:b]) <= 1e-2 * np.linalg.norm(gref[a:b]) + 1e-9, (a, b)
 
 
-def test_mlp_adam_matches_numpy(pkg, torch):
-    L = pkg._lib.load()
+def test_mlp_adam_matches_numpy(learned):
     rng = np.random.default_rng(5)
     p = rng.normal(0, 1, 5508).astype(np.float32)
-    pd = torch.tensor(p, device="cuda")
-    m = torch.zeros(5508, device="cuda")
-    v = torch.zeros(5508, device="cuda")
+    tr = learned.Trainer(16, p)
     pn, mn, vn = p.astype(np.float64), np.zeros(5508), np.zeros(5508)
     isw = np.zeros(5508, bool)
     isw[0:1024] = isw[1088:5184] = isw[5248:5504] = True
     lr, b1, b2, eps, wd = 1e-2, 0.9, 0.999, 1e-8, 1e-3
     for step in range(1, 4):
         gr = rng.normal(0, 1, 5508).astype(np.float32)
-        gd = torch.tensor(gr, device="cuda")
-        assert L.agimp_mlp_adam(_vp(pd), _vp(gd), _vp(m), _vp(v), step, lr, b1, b2, eps, wd, _stream(torch)) == 0
+        tr.apply(gr, lr, wd)
         mn = b1 * mn + (1 - b1) * gr
         vn = b2 * vn + (1 - b2) * gr.astype(np.float64) ** 2
         pn = np.where(isw, pn - lr * wd * pn, pn)
         pn = pn - lr * (mn / (1 - b1 ** step)) / (np.sqrt(vn / (1 - b2 ** step)) + eps)
-    torch.cuda.synchronize()
-    np.testing.assert_allclose(pd.cpu().numpy(), pn, rtol=1e-5, atol=1e-6)
+    got = tr.params()
+    tr.close()
+    np.testing.assert_allclose(got, pn, rtol=1e-5, atol=1e-6)
 
 
-def test_mlp_training_fits_a_teacher(pkg, torch, M):
-    """The library's forward/backward/Adam kernels train the network: a random teacher's
-    labels are fitted far better than chance after a few hundred steps."""
-    learned = __import__(pkg.__name__ + ".learned", fromlist=["Trainer"])
+def test_mlp_training_fits_a_teacher(learned, M):
+    """Epochs of the device-resident trainer (gradient + Adam kernels) fit a random teacher's
+    labels far better than chance (4 classes)."""
     rng = np.random.default_rng(6)
     teacher = M.init_params(7)
     X = rng.normal(0, 1, (32768, 16)).astype(np.float32)
     y = np.argmax(M.forward(teacher, X, bf16_rounding=False)[0], axis=1).astype(np.int32)
-    tr = learned.Trainer(M.init_params(8), lr=3e-3)
-    Xd, yd = torch.tensor(X, device="cuda"), torch.tensor(y, device="cuda")
-    first = None
-    for it in range(400):
-        idx = torch.randint(0, 32768, (4096,), device="cuda")
-        loss, _ = tr.grad(Xd[idx].contiguous(), yd[idx].contiguous())
-        if first is None:
-            first = loss.item()
-        tr.step()
-    acc = float(np.mean(np.argmax(tr.logits(Xd).cpu().numpy(), 1) == y))
-    print(f"teacher fit: first loss {first:.3f}, accuracy {acc:.3f}")
-    assert acc > 0.8
+    tr = learned.Trainer(len(y), M.init_params(8))
+    losses = []
+    for ep in range(12):
+        tr.set_data(X, y)
+        losses.append(tr.epoch(rng.permutation(len(y)), 1024, np.ones(4), 3e-3))
+    acc = float(np.mean(np.argmax(tr.logits(X), 1) == y))
+    tr.close()
+    print(f"teacher fit: losses {losses[0]:.3f} -> {losses[-1]:.3f}, accuracy {acc:.3f}")
+    assert losses[-1] < 0.5 * losses[0] and acc > 0.8
 
 
 # ----------------------------------------------------------------------------- unit features
@@ -245,6 +223,7 @@ def test_rollback_keeps_the_trajectory(pkg):
     sim = _single(pkg)
     sim.learned_enable(20, 0.3)
     sim.learned_set_model(_const_model(3))
+    sim.learned_set_parts(network=True, sweep=False)
     sim.advance_learned(dt, n)
     xl = sim.x_p.to_numpy()
     st = sim.learned_stats()
@@ -265,3 +244,23 @@ def test_finest_prediction_never_forces(pkg):
     print(st)
     assert st["forced"] == 0 and st["rebuilds"] == 3 and st["refined"] > 0
     assert c["nonfinite"] == 0
+
+
+def test_swept_requests_cover_straight_motion(pkg):
+    """With the swept requests the lone particle's window octree already holds every cell it
+    requests on its straight path: no rollback, same trajectory as the plain method."""
+    dt, n = 1e-4, 100
+    plain = _single(pkg)
+    plain.advance(dt, n)
+    xp = plain.x_p.to_numpy()
+    plain.close()
+    sim = _single(pkg)
+    sim.learned_enable(20, 0.3)
+    sim.learned_set_model(_const_model(3))
+    sim.learned_set_parts(network=False, sweep=True)
+    sim.advance_learned(dt, n)
+    st = sim.learned_stats()
+    xl = sim.x_p.to_numpy()
+    sim.close()
+    assert st["forced"] == 0 and st["rebuilds"] == 5
+    assert np.abs(xl - xp).max() < 1e-6

@@ -169,7 +169,8 @@ def test_features_hand_case():
     assert np.isclose(f[3], 1.0 * win)                               # mean |C|
     assert np.isclose(f[4], (1.331 - 1) / 2 * 10)                    # mean det F - 1, x10
     assert np.isclose(f[5], 0.0)                                     # min det F - 1 = 0
-    assert f[6] == 1.0 and f[7] == 0.75                              # max g = 2 (finest); mean g = 1
+    assert f[6] == 1.0                                               # max g = 2 (finest)
+    assert f[7] == 1.0                                               # moved by v * window, still requests level 2 here
     assert f[8] == 1.0                                               # the level-2 request is in this unit
     assert np.isclose(f[12], 2.0 * win * 16) and f[13] == 0 and f[14] == 0
     assert f[9] == 0 and f[10] == 0 and f[15] == 0                   # no occupied neighbours
