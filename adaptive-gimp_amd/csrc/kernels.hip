@@ -783,9 +783,11 @@ __device__ __forceinline__ float dpp_f(float src) {
 }
 
 // Which DPP steps of a segmented inclusive scan add their source: lanes sorted by key form
-// contiguous segments; a step adds only when the source lane carries the same key.
+// contiguous segments; a step adds only when the source lane carries the same key.  The scan
+// stays inside 16-lane rows: the last lane of each row piece of a segment ("tail") holds that
+// piece's sum (a segment spanning rows has one tail per row).
 struct SegMask {
-  bool s1, s2, s4, s8, b15, b31, tail;
+  bool s1, s2, s4, s8, tail;
 };
 
 __device__ __forceinline__ SegMask seg_mask(int key) {
@@ -794,22 +796,26 @@ __device__ __forceinline__ SegMask seg_mask(int key) {
   m.s2 = dpp_i<0x112, 0xf>(key, -1) == key;
   m.s4 = dpp_i<0x114, 0xf>(key, -1) == key;
   m.s8 = dpp_i<0x118, 0xf>(key, -1) == key;
-  m.b15 = dpp_i<0x142, 0xa>(key, -1) == key;
-  m.b31 = dpp_i<0x143, 0xc>(key, -1) == key;
   int nxt = __shfl_down(key, 1);
-  m.tail = (__lane_id() == 63) || (nxt != key);
+  m.tail = ((__lane_id() & 15) == 15) || (nxt != key);
   return m;
 }
 
-// Segmented inclusive scan: a segment's last lane ends up holding the segment's sum.
+// one step of the scan: lanes without a source read 0 (bound_ctrl), so the move folds into a
+// DPP add (v_add_f32_dpp) and the step costs the add and a select
+template <int CTRL>
+__device__ __forceinline__ float seg_step(float v, bool on) {
+  const float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+  const float s = v + t;
+  return on ? s : v;
+}
+
+// Segmented inclusive scan within rows: each tail ends up holding its row piece's sum.
 __device__ __forceinline__ float seg_sum(float v, const SegMask& m) {
-  float t;
-  t = dpp_f<0x111, 0xf>(v); v += m.s1 ? t : 0.f;
-  t = dpp_f<0x112, 0xf>(v); v += m.s2 ? t : 0.f;
-  t = dpp_f<0x114, 0xf>(v); v += m.s4 ? t : 0.f;
-  t = dpp_f<0x118, 0xf>(v); v += m.s8 ? t : 0.f;
-  t = dpp_f<0x142, 0xa>(v); v += m.b15 ? t : 0.f;
-  t = dpp_f<0x143, 0xc>(v); v += m.b31 ? t : 0.f;
+  v = seg_step<0x111>(v, m.s1);
+  v = seg_step<0x112>(v, m.s2);
+  v = seg_step<0x114>(v, m.s4);
+  v = seg_step<0x118>(v, m.s8);
   return v;
 }
 
@@ -833,8 +839,9 @@ constexpr int P2G_MAX_CH = 2048; // largest work chunk (particles of one leaf bl
 //  1. the chunk is counting-sorted by cell inside the block (LDS histogram); the order is
 //     written back to perm, so G2P stores particles cell-sorted for the next substep;
 //  2. each lane computes its particle's 3^d node contributions; lanes of one cell are summed
-//     with a segmented DPP scan, and only the cell's last lane adds into the (4+2)^d LDS node
-//     tile (no two lanes of one instruction hit the same LDS word);
+//     with a segmented DPP scan inside each 16-lane row, and only the last lane of each row
+//     piece adds into the (4+2)^d LDS node tile (at most 4 lanes of one instruction hit the
+//     same LDS word);
 //  3. the tile is flushed with one float atomic per non-zero node component.
 template <int D>
 __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
