@@ -146,7 +146,7 @@ enum Counter : int {
 constexpr int UNIT_ACC = 16;  // accumulator words per unit
 enum UnitAcc : int {
   UA_MASS = 0, UA_COUNT = 1, UA_VSUM = 2, UA_VMAX = 3, UA_CSUM = 4, UA_JSUM = 5, UA_JMIN = 6,  // JMIN: max key of -det F
-  UA_GMAX = 7, UA_GSUM = 8, UA_V = 9, UA_REQ = 12,  // GMAX, REQ: level + 1 (0: none); V: 3 words
+  UA_GMAX = 7, UA_V = 9, UA_REQ = 12,  // GMAX, REQ: level + 1 (0: none); V: 3 words
   UA_REQF = 13  // requests of the particles moved by v * window (level + 1)
 };
 constexpr int UNIT_NFEAT = 16;

@@ -123,9 +123,11 @@ __device__ __forceinline__ void record_requests(const Topo& T, int ulev, const f
     bool dup = q < 0;
 #pragma unroll
     for (int s = 0; s < t; ++s) dup = dup || seen[s] == q;
-    // most units already hold this level (many particles request the same units): a plain
-    // read first skips the contended atomic; a stale read only costs a redundant atomic
-    if (!dup && dst[q * stride] < g + 1) atomicMax(&dst[q * stride], g + 1);
+    // most units already hold this level (many particles request the same units): a read of
+    // the L2 value first (agent scope: a plain load may hit a stale per-CU cache line) skips
+    // the contended atomic; a stale read only costs a redundant atomic
+    if (!dup && __hip_atomic_load(&dst[q * stride], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g + 1)
+      atomicMax(&dst[q * stride], g + 1);
   }
 }
 
@@ -158,6 +160,28 @@ __global__ void k_unit_clear_labels(LearnDesc U) {
 }
 
 // ------------------------------------------------------------------------------ accumulation
+constexpr int ACC_TABLE_BITS = 8, ACC_TABLE = 1 << ACC_TABLE_BITS;  // >= threads per workgroup
+constexpr int ACC_NSUM = 8;  // mass, count, |v|, |C|, det F - 1, v (3)
+
+// add one unit's partial sums (order of `sums`: mass, count, |v|, |C|, det F - 1, v) and maxima
+// (|v| bits, key of -det F, level + 1) to its accumulators
+template <int D>
+__device__ __forceinline__ void flush_unit(const LearnDesc& U, int u, const float* sums, const uint32_t* maxs) {
+  float* af = reinterpret_cast<float*>(U.acc) + (int64_t)u * UNIT_ACC;
+  uint32_t* au = U.acc + (int64_t)u * UNIT_ACC;
+  atomicAdd(&af[UA_MASS], sums[0]);
+  atomicAdd(&af[UA_COUNT], sums[1]);
+  atomicAdd(&af[UA_VSUM], sums[2]);
+  atomicAdd(&af[UA_CSUM], sums[3]);
+  atomicAdd(&af[UA_JSUM], sums[4]);
+  atomicAdd(&af[UA_V + 0], sums[5]);
+  atomicAdd(&af[UA_V + 1], sums[6]);
+  if (D == 3) atomicAdd(&af[UA_V + 2], sums[7]);
+  atomicMax(&au[UA_VMAX], maxs[0]);
+  atomicMax(&au[UA_JMIN], maxs[1]);
+  atomicMax(reinterpret_cast<int*>(&au[UA_GMAX]), (int)maxs[2]);
+  U.uflag[u] = 1;
+}
 template <int D>
 __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, LearnDesc U) {
   const SceneDesc& sd = Dsc[0];
@@ -218,7 +242,7 @@ __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, Le
   const bool in = live && u >= 0;
   const int key = in ? u : -2 - (int)__lane_id();
   const USeg sm = useg(key);
-  float s_m = m, s_n = in ? 1.f : 0.f, s_v = vn, s_c = cn, s_j = jm1, s_g = (float)g;
+  float s_m = m, s_n = in ? 1.f : 0.f, s_v = vn, s_c = cn, s_j = jm1;
   float s_v0 = v[0], s_v1 = v[1], s_v2 = v[2];
   float x_v = vn, x_j = njac, x_g = (float)(g + 1);
   USEG_STEP(op_add, s_m, sm);
@@ -226,29 +250,56 @@ __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, Le
   USEG_STEP(op_add, s_v, sm);
   USEG_STEP(op_add, s_c, sm);
   USEG_STEP(op_add, s_j, sm);
-  USEG_STEP(op_add, s_g, sm);
   USEG_STEP(op_add, s_v0, sm);
   USEG_STEP(op_add, s_v1, sm);
   if (D == 3) USEG_STEP(op_add, s_v2, sm);
   USEG_STEP(op_max, x_v, sm);
   USEG_STEP(op_max, x_j, sm);
   USEG_STEP(op_max, x_g, sm);
+  // run sums of the workgroup are merged per unit in an LDS table first (units of one
+  // workgroup are few, but their runs are short: particles are ordered by base cell, units
+  // by containing cell), then each unit is flushed once with global atomics
+  __shared__ int tkey[ACC_TABLE];
+  __shared__ float tsum[ACC_TABLE][ACC_NSUM];
+  __shared__ uint32_t tmax[ACC_TABLE][3];
+  for (int i = threadIdx.x; i < ACC_TABLE; i += blockDim.x) {
+    tkey[i] = -1;
+#pragma unroll
+    for (int j = 0; j < ACC_NSUM; ++j) tsum[i][j] = 0.f;
+    tmax[i][0] = tmax[i][1] = tmax[i][2] = 0u;
+  }
+  __syncthreads();
   if (in && sm.tail) {
-    float* af = reinterpret_cast<float*>(U.acc) + (int64_t)u * UNIT_ACC;
-    uint32_t* au = U.acc + (int64_t)u * UNIT_ACC;
-    atomicAdd(&af[UA_MASS], s_m);
-    atomicAdd(&af[UA_COUNT], s_n);
-    atomicAdd(&af[UA_VSUM], s_v);
-    atomicAdd(&af[UA_CSUM], s_c);
-    atomicAdd(&af[UA_JSUM], s_j);
-    atomicAdd(&af[UA_GSUM], s_g);
-    atomicAdd(&af[UA_V + 0], s_v0);
-    atomicAdd(&af[UA_V + 1], s_v1);
-    if (D == 3) atomicAdd(&af[UA_V + 2], s_v2);
-    atomicMax(&au[UA_VMAX], __float_as_uint(x_v));
-    atomicMax(&au[UA_JMIN], fkey(x_j));
-    atomicMax(reinterpret_cast<int*>(&au[UA_GMAX]), (int)x_g);
-    U.uflag[u] = 1;
+    int slot = -1;
+    uint32_t h = ((uint32_t)u * 2654435761u) >> (32 - ACC_TABLE_BITS);
+    for (int probe = 0; probe < ACC_TABLE; ++probe) {
+      const int e = (int)((h + probe) & (ACC_TABLE - 1));
+      const int prev = atomicCAS(&tkey[e], -1, u);
+      if (prev == -1 || prev == u) { slot = e; break; }
+    }
+    const float sums[ACC_NSUM] = {s_m, s_n, s_v, s_c, s_j, s_v0, s_v1, s_v2};
+    const uint32_t maxs[3] = {__float_as_uint(x_v), fkey(x_j), (uint32_t)(int)x_g};
+    if (slot >= 0) {
+#pragma unroll
+      for (int j = 0; j < ACC_NSUM; ++j)
+        if (j < 7 || D == 3) atomicAdd(&tsum[slot][j], sums[j]);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) atomicMax(&tmax[slot][j], maxs[j]);
+    } else {  // table full (not expected for 256 particles): straight to the unit
+      flush_unit<D>(U, u, sums, maxs);
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ACC_TABLE; i += blockDim.x) {
+    const int uu = tkey[i];
+    if (uu < 0) continue;
+    float sums[ACC_NSUM];
+    uint32_t maxs[3];
+#pragma unroll
+    for (int j = 0; j < ACC_NSUM; ++j) sums[j] = tsum[i][j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) maxs[j] = tmax[i][j];
+    flush_unit<D>(U, uu, sums, maxs);
   }
 }
 

@@ -51,10 +51,10 @@ def algorithmic_bytes(dim, n_particles, n_nodes):
     return p2g, g2p
 
 
-def _oracle_for(wl, threads):
+def _oracle_for(wl, threads, fp64=False):
     from oracle import oracle as O
 
-    ora = O.OracleSim(wl.dim, wl.level, wl.coarsest_size, wl.n, fp64=False, threads=threads, **wl.params)
+    ora = O.OracleSim(wl.dim, wl.level, wl.coarsest_size, wl.n, fp64=fp64, threads=threads, **wl.params)
     st = wl.state
     ora.set_particles(st["x"], st["v"], st["F"], st["C"], st["m"], st["g"])
     if wl.grid_initializer is not None:
@@ -98,6 +98,82 @@ def cpu_baseline(workloads, seconds_target=12.0):
 METRIC = "particle-substeps/s (3D 1M p, 3 levels) plain vs learned-adapt; HBM GB/s"
 
 
+def _threads():
+    t = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    return max(1, min(t, 16))
+
+
+def _new_sim(pkg, wl, ppc):
+    return pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(), wl.grid_initializer,
+                            particles_per_chunk=ppc, **wl.params)
+
+
+def _learned_sim(pkg, wl, ppc, model, parts):
+    sim = _new_sim(pkg, wl, ppc)
+    sim.learned_enable(model[1].get("window", 20), 0.3)
+    sim.learned_set_model(model[0])
+    sim.learned_set_parts(network="network" in parts, sweep="sweep" in parts)
+    return sim
+
+
+def bench_learned(pkg, wl, args, np, model, plain_x):
+    """The same workload with predicted octrees (agimp_learned_advance): throughput over the
+    same warmup + steps from the same initial state, rebuild statistics, and the positions'
+    deviation from the plain run at the end."""
+    sim = _learned_sim(pkg, wl, args.ppc, model, args.learned_parts)
+    sim.advance_learned(wl.dt, args.warmup)
+    sim.sync()
+    sim.learned_reset_stats()
+    t0 = time.perf_counter()
+    sim.advance_learned(wl.dt, args.steps)
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    stats = sim.learned_stats()
+    dev = np.linalg.norm(sim.x_p.to_numpy().astype(np.float64) - plain_x, axis=1)
+    counters = sim.counters()
+    sim.close()
+    return elapsed, stats, dev, counters
+
+
+def accuracy(pkg, wl, args, np, model):
+    """Positions and velocities of the plain and the learned GPU paths against the FP64 oracle
+    (float64 particles and grid, FP64 SVD) after `n` substeps from the same initial state."""
+    n = args.accuracy_steps
+    ora = _oracle_for(wl, _threads(), fp64=True)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        ora.substep(wl.dt)
+    t_ora = time.perf_counter() - t0
+    ref = ora.get_particles()
+    mass0 = float(np.sum(wl.state["m"]))
+    out = {"substeps": n, "reference": "oracle FP64 (C++ restatement, double particles and grid)",
+           "oracle_seconds": round(t_ora, 1)}
+    runs = [("plain", None)] + ([("learned", model)] if model is not None else [])
+    for name, mdl in runs:
+        if mdl is None:
+            sim = _new_sim(pkg, wl, args.ppc)
+            for _ in range(n):
+                sim.substep(wl.dt)
+        else:
+            sim = _learned_sim(pkg, wl, args.ppc, mdl, args.learned_parts)
+            sim.advance_learned(wl.dt, n)
+        x = sim.x_p.to_numpy().astype(np.float64)
+        v = sim.v_p.to_numpy().astype(np.float64)
+        F = sim.F_p.to_numpy().astype(np.float64)
+        sim.close()
+        ex = np.linalg.norm(x - ref["x"], axis=1)
+        vs = max(float(np.abs(ref["v"]).max()), 1e-30)
+        mom = np.sum(wl.state["m"][:, None] * v, axis=0)
+        mom_ref = np.sum(wl.state["m"][:, None] * ref["v"], axis=0)
+        ev = np.linalg.norm(v - ref["v"], axis=1) / vs
+        out[name] = {"x_max_abs": float(ex.max()), "x_p99_abs": float(np.percentile(ex, 99)),
+                     "v_max_rel": float(ev.max()), "v_p99_rel": float(np.percentile(ev, 99)),
+                     "F_max_abs": float(np.abs(F - ref["F"]).max()),
+                     "momentum_rel": float(np.linalg.norm(mom - mom_ref) / max(np.linalg.norm(mom_ref), 1e-30)),
+                     "mass": "exact (particle masses are constant)" if mass0 > 0 else None}
+    return out
+
+
 def bench_single(pkg, wl, args, np):
     sim = pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(), wl.grid_initializer,
                            particles_per_chunk=args.ppc, **wl.params)
@@ -123,7 +199,12 @@ def bench_single(pkg, wl, args, np):
     n_nodes = int(sum(int(np.count_nonzero(sim.ad_grid_m[l].to_numpy())) for l in range(wl.level)))
     counters = sim.counters()
     sim.close()
-    return elapsed, tim, n_nodes, counters, tuned
+    # positions after warmup + steps substeps from the initial state (for the learned comparison)
+    ref = _new_sim(pkg, wl, tuned[0] if tuned else args.ppc)
+    ref.advance(wl.dt, args.warmup + args.steps)
+    plain_x = ref.x_p.to_numpy().astype(np.float64)
+    ref.close()
+    return elapsed, tim, n_nodes, counters, tuned, plain_x
 
 
 def bench_batch(pkg, wls, args):
@@ -161,6 +242,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the P2G chunk size first")
     ap.add_argument("--ppc", type=int, default=1024, help="P2G particles per chunk (single-scene workloads)")
+    ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"],
+                    help="single-scene workloads: time the plain path, the learned path (predicted octrees, "
+                         "adaptive-gimp_amd/refine_mlp.npz), or both (value = the plain path)")
+    ap.add_argument("--learned-parts", default="network,sweep",
+                    help="parts of a predicted octree: network (unit levels), sweep (velocity-swept requests)")
+    ap.add_argument("--accuracy-steps", type=int, default=-1,
+                    help="substeps of the FP64-oracle accuracy leg (default: 20 for c3, 0 = off otherwise)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run one plain device copy of this many MiB (a known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -203,7 +291,13 @@ def main():
             "dtype": "f32 (FP64 SVD/plasticity)", "data": "synthetic (seeded scene generators)"}
     if len(wls) == 1:
         wl = wls[0]
-        elapsed, tim, n_nodes, counters, tuned = bench_single(pkg, wl, args, np)
+        model = None
+        if args.mode in ("learned", "both") and wl.grid_initializer is None:
+            from importlib import import_module
+
+            lm = import_module(pkg.__name__ + ".learned")
+            model = lm.load_model()
+        elapsed, tim, n_nodes, counters, tuned, plain_x = bench_single(pkg, wl, args, np)
         b_p2g, b_g2p = algorithmic_bytes(wl.dim, wl.n, n_nodes)
         avg = {k: (v[0] / v[1] if v[1] else float("nan")) for k, v in tim.items()}
         kern = {"p2g": (b_p2g, avg["p2g"]), "g2p": (b_g2p, avg["g2p"])}
@@ -226,6 +320,27 @@ def main():
         })
         if tuned:
             line["autotune_s_per_substep"] = {str(k): v for k, v in tuned[1].items()}
+        if model is not None:
+            el, st, dev, lcnt = bench_learned(pkg, wl, args, np, model, plain_x)
+            lv = total_n * args.steps / el
+            line["learned"] = {
+                "value": lv, "ms_per_step": el * 1e3 / args.steps, "speedup_vs_plain": lv / line["value"],
+                "parts": args.learned_parts, "window": model[1].get("window", 20), "tau": 0.3,
+                "stats": st, "criterion_failures_executed": 0,
+                "criterion_failures_detected": st["violations"],
+                "x_dev_from_plain": {"max": float(dev.max()), "p99": float(np.percentile(dev, 99)),
+                                     "after_substeps": args.warmup + args.steps},
+                "counters": lcnt,
+            }
+            if args.mode == "learned":
+                line["value_plain"] = line["value"]
+                line["value"] = lv
+                line["ms_per_step"] = el * 1e3 / args.steps
+                line["config"]["mode"] = "learned"
+        acc_steps = args.accuracy_steps if args.accuracy_steps >= 0 else (20 if wl.name.startswith("c3") else 0)
+        if acc_steps > 0 and not args.no_cpu:
+            args.accuracy_steps = acc_steps
+            line["accuracy"] = accuracy(pkg, wl, args, np, model)
     else:
         elapsed, counters, tuned = bench_batch(pkg, wls, args)
         line.update({
