@@ -431,10 +431,10 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   if (rc) { agimp_destroy(h); return rc; }
   size_t co = 0, fo = 0;
   for (int l = 0; l < T.L; ++l) {
-    T.lv[l].cell = cellbuf + co;
-    T.lv[l].grid = gridbuf + co;
-    T.lv[l].sflag = flagbuf + fo;
-    T.lv[l].slist = slistbuf + fo;
+    T.lv[l].cell = as_global(cellbuf + co);
+    T.lv[l].grid = as_global(gridbuf + co);
+    T.lv[l].sflag = as_global(flagbuf + fo);
+    T.lv[l].slist = as_global(slistbuf + fo);
     co += (size_t)T.lv[l].nblocks * p4;
     fo += (size_t)T.lv[l].nblocks;
   }
@@ -555,8 +555,8 @@ int agimp_set_static_levels(agimp_handle* h, const int8_t* ret_level, const int8
     int8_t *r = nullptr, *a = nullptr;
     HIPCHK(h, hipMalloc(&r, cnt));
     HIPCHK(h, hipMalloc(&a, cnt));
-    h->hd.st_ret = r;
-    h->hd.st_act = a;
+    h->hd.st_ret = as_global(r);
+    h->hd.st_act = as_global(a);
   }
   HIPCHK(h, hipMemcpyAsync((void*)h->hd.st_ret, ret_level, cnt, hipMemcpyHostToDevice, h->stream));
   HIPCHK(h, hipMemcpyAsync((void*)h->hd.st_act, act_level, cnt, hipMemcpyHostToDevice, h->stream));
@@ -772,7 +772,7 @@ int agimp_learned_enable(agimp_handle* h, int32_t window, float tau) {
     rc = rc ? rc : dalloc(h, &h->snap_cnt, NCOUNTERS);
     if (rc) return rc;
     HIPCHK(h, hipMemsetAsync(U.upred, 0xff, nu4, h->stream));
-    U.params = params;
+    U.params = as_global(params);
     double vol = 1.0;
     for (int a = 0; a < h->D; ++a) vol *= (double)Lu.dx;
     U.inv_mref = (float)((double)h->cfg.p_vol / (h->mean_mass * vol));

@@ -12,7 +12,23 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Device buffers live in global memory.  Typing the descriptors' pointers as such in the
+// device pass makes every access through them a global_* instruction (SGPR base + VGPR
+// offset, waited on by vmcnt alone) instead of a flat_* one, whose lgkmcnt share would
+// make every scalar load of a descriptor field wait for all outstanding vector loads.
+#ifdef __HIP_DEVICE_COMPILE__
+#define AGIMP_GLOBAL __attribute__((address_space(1)))
+#else
+#define AGIMP_GLOBAL
+#endif
+
 namespace agimp {
+
+// a device allocation as a descriptor pointer
+template <class T>
+__host__ __device__ inline T AGIMP_GLOBAL* as_global(T* p) {
+  return (T AGIMP_GLOBAL*)p;
+}
 
 constexpr int MAXL = 6;  // octree levels supported
 constexpr int PAD = 4;   // padding cells on each side of every level (one leaf block)
@@ -32,10 +48,10 @@ struct LevelDesc {
   int nblocks;  // NB^d
   int gbase;    // first global block id of this level
   float dx, inv_dx;
-  uint8_t* cell;   // [nblocks * 4^d]
-  float4* grid;    // [nblocks * 4^d]
-  uint8_t* sflag;  // [nblocks]   block must be processed by the grid phase
-  int* slist;      // [nblocks]   compacted sflag list
+  uint8_t AGIMP_GLOBAL* cell;   // [nblocks * 4^d]
+  float4 AGIMP_GLOBAL* grid;    // [nblocks * 4^d]
+  uint8_t AGIMP_GLOBAL* sflag;  // [nblocks]   block must be processed by the grid phase
+  int AGIMP_GLOBAL* slist;      // [nblocks]   compacted sflag list
 };
 
 struct Topo {
@@ -44,11 +60,11 @@ struct Topo {
   int total_blocks;  // sum of nblocks over levels
   int pow2;          // coarsest size is a power of two: x/dx == x*inv_dx exactly
   LevelDesc lv[MAXL];
-  uint8_t* node;     // node_mask, finest layout
-  int* scount;       // [MAXL] number of entries in each slist
-  uint8_t* cflag;    // [total_blocks] leaf block holds a non-zero cell (by global block id)
-  int* clist;        // [total_blocks] compacted cflag (global block ids)
-  int* ccount;       // [4] list lengths of the successive compactions of cflag
+  uint8_t AGIMP_GLOBAL* node;     // node_mask, finest layout
+  int AGIMP_GLOBAL* scount;       // [MAXL] number of entries in each slist
+  uint8_t AGIMP_GLOBAL* cflag;    // [total_blocks] leaf block holds a non-zero cell (by global block id)
+  int AGIMP_GLOBAL* clist;        // [total_blocks] compacted cflag (global block ids)
+  int AGIMP_GLOBAL* ccount;       // [4] list lengths of the successive compactions of cflag
 };
 
 struct Physics {
@@ -81,25 +97,25 @@ __host__ __device__ inline int tau_idx(int i, int j) {
 }
 
 struct PBuf {
-  float* f;       // [NF * stride]
-  int* g;         // [stride] prescribed level g_p
-  int* id;        // [stride] original particle index
+  float AGIMP_GLOBAL* f;       // [NF * stride]
+  int AGIMP_GLOBAL* g;         // [stride] prescribed level g_p
+  int AGIMP_GLOBAL* id;        // [stride] original particle index
   int64_t stride;
 };
 
 // scratch of the (level, leaf block) counting sort and the P2G chunk list
 struct SortBufs {
-  int* gb;       // [n]  global block id of each slot's particle (-1: frozen)
-  int* rk;       // [n]  rank inside its block (or among frozen particles)
-  int* perm;     // [n]  sorted position -> slot
-  int* flv;      // [n]  fl_p of the particle now in each slot
-  int* bcnt;     // [total_blocks] particles per block
-  int* boff;     // [total_blocks] first sorted position of each block
-  int* choff;    // [total_blocks] first chunk of each block
-  int2* tiles;   // scan tile partials
-  int* totals;   // [2] active particles, chunks
-  int* frozen;   // [1] frozen-particle counter
-  int4* chunks;  // (gbid, first position, count, -)
+  int AGIMP_GLOBAL* gb;       // [n]  global block id of each slot's particle (-1: frozen)
+  int AGIMP_GLOBAL* rk;       // [n]  rank inside its block (or among frozen particles)
+  int AGIMP_GLOBAL* perm;     // [n]  sorted position -> slot
+  int AGIMP_GLOBAL* flv;      // [n]  fl_p of the particle now in each slot
+  int AGIMP_GLOBAL* bcnt;     // [total_blocks] particles per block
+  int AGIMP_GLOBAL* boff;     // [total_blocks] first sorted position of each block
+  int AGIMP_GLOBAL* choff;    // [total_blocks] first chunk of each block
+  int2 AGIMP_GLOBAL* tiles;   // scan tile partials
+  int AGIMP_GLOBAL* totals;   // [2] active particles, chunks
+  int AGIMP_GLOBAL* frozen;   // [1] frozen-particle counter
+  int4 AGIMP_GLOBAL* chunks;  // (gbid, first position, count, -)
 };
 
 // Everything the kernels need about one scene.  Kernels take an array of these and pick
@@ -114,10 +130,10 @@ struct SceneDesc {
   int n;
   int static_mode;
   int CH;  // P2G particles per work chunk
-  const int8_t* st_ret;
-  const int8_t* st_act;
-  unsigned long long* cnt;
-  uint32_t* vflag;
+  const int8_t AGIMP_GLOBAL* st_ret;
+  const int8_t AGIMP_GLOBAL* st_act;
+  unsigned long long AGIMP_GLOBAL* cnt;
+  uint32_t AGIMP_GLOBAL* vflag;
   size_t w_cells, w_node, w_flags;  // 32-bit words of the cell masks, node mask, block flags
   // refinement criterion check (used while a predicted octree is reused): a particle's 2^d
   // request cells at level g_p must be covered by cells of level >= g_p, i.e. hold the
@@ -125,7 +141,7 @@ struct SceneDesc {
   // upred >= g_p
   int check;
   int ulev;
-  const int8_t* upred;
+  const int8_t AGIMP_GLOBAL* upred;
 };
 
 // counters kept on the device (int64 each)
@@ -158,16 +174,16 @@ struct LearnDesc {
   float inv_mref;      // 1 / (reference density * unit volume)
   float tau;           // decision threshold on the cumulative class probability
   int mode;            // parts of a predicted octree: 1 network unit levels, 2 swept requests
-  uint32_t* acc;       // [nu * UNIT_ACC]
-  uint8_t* uflag;      // [nu] unit holds particles
-  uint8_t* cflag;      // [nu] unit is a candidate (within one unit of an occupied one)
-  int* occ;            // [nu] compacted occupied units
-  int* cand;           // [nu] compacted candidates
-  int* counts;         // [4] occupied, candidates, units refined by the prediction, -
-  float* feat;         // [nu * UNIT_NFEAT] features of the candidates (candidate order)
-  int* lab;            // [nu] finest requested level + 1 over a label window
-  int8_t* upred;       // [nu] predicted level (-1: none)
-  const float* params; // network parameters (5508 floats)
+  uint32_t AGIMP_GLOBAL* acc;       // [nu * UNIT_ACC]
+  uint8_t AGIMP_GLOBAL* uflag;      // [nu] unit holds particles
+  uint8_t AGIMP_GLOBAL* cflag;      // [nu] unit is a candidate (within one unit of an occupied one)
+  int AGIMP_GLOBAL* occ;            // [nu] compacted occupied units
+  int AGIMP_GLOBAL* cand;           // [nu] compacted candidates
+  int AGIMP_GLOBAL* counts;         // [4] occupied, candidates, units refined by the prediction, -
+  float AGIMP_GLOBAL* feat;         // [nu * UNIT_NFEAT] features of the candidates (candidate order)
+  int AGIMP_GLOBAL* lab;            // [nu] finest requested level + 1 over a label window
+  int8_t AGIMP_GLOBAL* upred;       // [nu] predicted level (-1: none)
+  const float AGIMP_GLOBAL* params; // network parameters (5508 floats)
 };
 
 // block-major index of padded coordinates J (each in [0, E))

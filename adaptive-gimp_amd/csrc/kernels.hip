@@ -207,7 +207,7 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
   if (k >= n) return;
   int g = P.g[k];
   if (g < 0 || g >= T.L) return;
-  const LevelDesc& Lg = T.lv[g];
+  const LevelDesc Lg = T.lv[g];
   int b[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) b[a] = base_cell(P.f[(size_t)a * P.stride + k], Lg, T.pow2);
@@ -227,7 +227,7 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
 template <int D>
 __device__ __forceinline__ int static_gi(const Topo& T, const int8_t* ret, const int8_t* act, const int* I,
                                          unsigned long long* cnt) {
-  const LevelDesc& F = T.lv[T.L - 1];
+  const LevelDesc F = T.lv[T.L - 1];
   int J[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) J[a] = I[a] + PAD;
@@ -262,7 +262,7 @@ __global__ void k_activate_static(const SceneDesc* __restrict__ Dsc, int par) {
   unsigned long long* cnt = sd.cnt;
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const LevelDesc& F = T.lv[T.L - 1];
+  const LevelDesc F = T.lv[T.L - 1];
   float x[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
@@ -501,8 +501,8 @@ template <int D>
 __global__ void k_cover(const SceneDesc* __restrict__ Dsc, int l) {
   AGIMP_SCENE
   if (l + 1 >= T.L) return;
-  const LevelDesc& Lc = T.lv[l + 1];
-  const LevelDesc& Lp = T.lv[l];
+  const LevelDesc Lc = T.lv[l + 1];
+  const LevelDesc Lp = T.lv[l];
   const int64_t tot = (int64_t)Lc.nblocks << D;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(t >> D);
@@ -565,8 +565,14 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
     for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
     const int gp = sd.check ? P.g[k] : -1;  // level whose request cells are checked
     bool viol = false;
-    for (int l = 0; l < T.L; ++l) {
-      const LevelDesc& Ll = T.lv[l];
+    // from the finest level down: the first level with an activated cell is fl; below it
+    // only the checked level gp (if any) still needs its cells
+    for (int l = T.L - 1; l >= 0; --l) {
+      if (fl >= 0 && l != gp) {
+        if (gp >= 0 && l > gp) continue;
+        break;
+      }
+      const LevelDesc Ll = T.lv[l];
       int b[3];
 #pragma unroll
       for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Ll, T.pow2);
@@ -582,14 +588,14 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
         if (l == gp && q >= 0 && !(c & (ACTIVATED | COVERED)))
           viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
       }
-      if (hit) fl = l;
+      if (hit && fl < 0) fl = l;
     }
     if (viol) {
       count(cnt, CNT_VIOL);
       atomicMin(&cnt[CNT_FIRSTVIOL], cnt[CNT_STEP]);
     }
     if (fl >= 0) {
-      const LevelDesc& Lf = T.lv[fl];
+      const LevelDesc Lf = T.lv[fl];
       int J[3];
       bool ok = true;
 #pragma unroll
@@ -734,7 +740,7 @@ __global__ void k_blocks(const SceneDesc* __restrict__ Dsc) {
   int c = bcnt[g];
   if (c == 0) return;
   int l = level_of_gbid(T, g);
-  const LevelDesc& Ll = T.lv[l];
+  const LevelDesc Ll = T.lv[l];
   int b = g - Ll.gbase;
   int nch = (c + CH - 1) / CH;
   int o = choff[g], s = boff[g];
@@ -865,7 +871,7 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int4 ck = chunks[c];
     const int l = level_of_gbid(T, ck.x);
-    const LevelDesc& Ll = T.lv[l];
+    const LevelDesc Ll = T.lv[l];
     int B[3];
     block_coords<D>(Ll.NB, ck.x - Ll.gbase, B);
     for (int i = tid; i < TN * 4; i += P2G_T) tile[i] = 0.f;
@@ -1037,6 +1043,24 @@ __device__ __forceinline__ uint8_t nflag(const Topo& T, int l, const int* I) {
   int64_t q = nidx<D>(T, l, I);
   return q < 0 ? (uint8_t)0 : T.node[q];
 }
+// the finest node mask and its extents, read once per kernel into scalar registers
+struct NodeView {
+  const uint8_t AGIMP_GLOBAL* node;
+  int NB, E;
+};
+__device__ __forceinline__ NodeView node_view(const Topo& T) {
+  const LevelDesc Ln = T.lv[T.L - 1];
+  return NodeView{T.node, Ln.NB, Ln.E};
+}
+// node flags of the node I of the level whose index shift to the finest level is `shift`
+template <int D>
+__device__ __forceinline__ uint8_t nflag(const NodeView& N, int shift, const int* I) {
+  int J[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) J[a] = (I[a] << shift) + PAD;
+  if (!in_box<D>(N.E, J)) return 0;
+  return N.node[bidx<D>(N.NB, J)];
+}
 __device__ __forceinline__ bool f_valid(uint8_t M) { return M > 0; }
 __device__ __forceinline__ bool f_act(uint8_t M) { return (M & (ACTIVATED | T_JUNCTION)) == ACTIVATED; }
 __device__ __forceinline__ bool f_T(uint8_t M) { return (M & T_JUNCTION) != 0; }
@@ -1073,31 +1097,49 @@ template <int D>
 __global__ void k_restrict(const SceneDesc* __restrict__ Dsc, int l) {
   AGIMP_SCENE
   if (l >= T.L) return;
-  const LevelDesc& Lc = T.lv[l - 1];
-  const LevelDesc& Lf = T.lv[l];
+  const LevelDesc Lc = T.lv[l - 1];
+  const LevelDesc Lf = T.lv[l];
+  const NodeView N = node_view(T);
   int total = T.scount[l - 1] * Pow4<D>::v;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     int I[3];
     uint32_t q = list_node<D>(Lc, t / Pow4<D>::v, t % Pow4<D>::v, I);
     if (!in_range_nodes<D>(I, Lc.n)) continue;
-    if (!f_valid(nflag<D>(T, l - 1, I))) continue;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!f_valid(nflag<D>(N, Lc.shift, I))) continue;
+    // the node flags of all sources first, then their values: independent loads in flight
+    // together instead of one flag -> value round trip per source
     constexpr int NS = (D == 2) ? 9 : 27;
-    bool any = false;
-    for (int s = 0; s < NS; ++s) {
-      int d[3], If[3], r = s, l1 = 0;
+    int qf[NS];
+    uint8_t M[NS];
 #pragma unroll
-      for (int a = 0; a < D; ++a) { d[a] = (r % 3) - 1; r /= 3; If[a] = 2 * I[a] - d[a]; l1 += d[a] != 0; }
-      if (!in_range_nodes<D>(If, Lf.n)) continue;
-      uint8_t M = nflag<D>(T, l, If);
-      bool cond = (l1 == 0 && f_act(M)) || f_T(M) || f_ghost(M);
-      if (!cond) continue;
-      int64_t qf = lidx<D>(Lf, If);
-      if (qf < 0) continue;
-      float4 val = Lf.grid[qf];
-      float wgt = (l1 == 0) ? 1.f : (l1 == 1 ? 0.5f : (l1 == 2 ? 0.25f : 0.125f));
-      acc.x += val.x * wgt; acc.y += val.y * wgt; acc.z += val.z * wgt; acc.w += val.w * wgt;
-      any = true;
+    for (int s = 0; s < NS; ++s) {
+      int If[3], r = s;
+#pragma unroll
+      for (int a = 0; a < D; ++a) { If[a] = 2 * I[a] - ((r % 3) - 1); r /= 3; }
+      qf[s] = -1;
+      M[s] = 0;
+      if (in_range_nodes<D>(If, Lf.n)) {
+        const int64_t qq = lidx<D>(Lf, If);
+        if (qq >= 0) {
+          qf[s] = (int)qq;
+          M[s] = nflag<D>(N, Lf.shift, If);
+        }
+      }
+    }
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      int r = s, l1 = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) { l1 += (r % 3) != 1; r /= 3; }
+      const bool cond = qf[s] >= 0 && ((l1 == 0 && f_act(M[s])) || f_T(M[s]) || f_ghost(M[s]));
+      if (cond) {
+        const float4 val = Lf.grid[qf[s]];
+        const float wgt = (l1 == 0) ? 1.f : (l1 == 1 ? 0.5f : (l1 == 2 ? 0.25f : 0.125f));
+        acc.x += val.x * wgt; acc.y += val.y * wgt; acc.z += val.z * wgt; acc.w += val.w * wgt;
+        any = true;
+      }
     }
     if (!any) continue;
     float4 g = Lc.grid[q];
@@ -1118,7 +1160,7 @@ __global__ void k_gridop(const SceneDesc* __restrict__ Dsc, float dt_arg) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     int l = 0, r = t;
     while (r >= tot[l]) { r -= tot[l]; ++l; }
-    const LevelDesc& Ll = T.lv[l];
+    const LevelDesc Ll = T.lv[l];
     int I[3];
     uint32_t q = list_node<D>(Ll, r / Pow4<D>::v, r % Pow4<D>::v, I);
     if (!in_range_nodes<D>(I, Ll.n)) continue;
@@ -1163,14 +1205,15 @@ template <int D>
 __global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l) {
   AGIMP_SCENE
   if (l >= T.L) return;
-  const LevelDesc& Lf = T.lv[l];
-  const LevelDesc& Lc = T.lv[l - 1];
+  const LevelDesc Lf = T.lv[l];
+  const LevelDesc Lc = T.lv[l - 1];
+  const NodeView N = node_view(T);
   int total = T.scount[l] * Pow4<D>::v;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
     int J[3];
     uint32_t q = list_node<D>(Lf, t / Pow4<D>::v, t % Pow4<D>::v, J);
     if (!in_range_nodes<D>(J, Lf.n)) continue;
-    uint8_t M = nflag<D>(T, l, J);
+    uint8_t M = nflag<D>(N, Lf.shift, J);
     bool isT = f_T(M), isG = f_ghost(M), isA = f_act(M);
     bool even = true;
 #pragma unroll
@@ -1181,23 +1224,38 @@ __global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l) {
       int nodd = 0;
 #pragma unroll
       for (int a = 0; a < D; ++a) nodd += (J[a] & 1);
-      int combos = 1 << nodd;
-      for (int u = 0; u < combos; ++u) {
-        int Ic[3], bit = 0, l1 = 0;
+      const int combos = (isT || isG) ? 1 << nodd : (nodd == 0 ? 1 : 0);
+      const float wgt = (nodd == 0) ? 1.f : (nodd == 1 ? 0.5f : (nodd == 2 ? 0.25f : 0.125f));
+      // all source flags first, then the values (independent loads in flight together)
+      int qc[1 << D];
+      uint8_t Mc[1 << D];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-          int d = 0;
-          if (J[a] & 1) { d = ((u >> bit) & 1) ? 1 : -1; ++bit; ++l1; }
-          Ic[a] = (J[a] - d) >> 1;
+      for (int u = 0; u < (1 << D); ++u) {
+        qc[u] = -1;
+        Mc[u] = 0;
+        if (u < combos) {
+          int Ic[3], bit = 0;
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            int d = 0;
+            if (J[a] & 1) { d = ((u >> bit) & 1) ? 1 : -1; ++bit; }
+            Ic[a] = (J[a] - d) >> 1;
+          }
+          if (in_range_nodes<D>(Ic, Lc.n)) {
+            const int64_t qq = lidx<D>(Lc, Ic);
+            if (qq >= 0) {
+              qc[u] = (int)qq;
+              Mc[u] = nflag<D>(N, Lc.shift, Ic);
+            }
+          }
         }
-        if (l1 > 0 && !(isT || isG)) continue;
-        if (!in_range_nodes<D>(Ic, Lc.n)) continue;
-        if (!f_valid(nflag<D>(T, l - 1, Ic))) continue;
-        int64_t qc = lidx<D>(Lc, Ic);
-        if (qc < 0) continue;
-        float4 vc = Lc.grid[qc];
-        float wgt = (l1 == 0) ? 1.f : (l1 == 1 ? 0.5f : (l1 == 2 ? 0.25f : 0.125f));
-        acc[0] += vc.x * wgt; acc[1] += vc.y * wgt; acc[2] += vc.z * wgt;
+      }
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        if (qc[u] >= 0 && f_valid(Mc[u])) {
+          const float4 vc = Lc.grid[qc[u]];
+          acc[0] += vc.x * wgt; acc[1] += vc.y * wgt; acc[2] += vc.z * wgt;
+        }
       }
     }
     float4 g = Lf.grid[q];
@@ -1234,7 +1292,7 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
     return;
   }
   int l = level_of_gbid(T, g);
-  const LevelDesc& Ll = T.lv[l];
+  const LevelDesc Ll = T.lv[l];
   float x[3], w[3][3], dw[3][3];
   int J[3];
 #pragma unroll
@@ -1411,7 +1469,7 @@ __global__ void k_download(const SceneDesc* __restrict__ Dsc, int par, float* x,
 template <int D>
 __global__ void k_dense_level(const SceneDesc* __restrict__ Dsc, int l, int* cell, float* m, float* v) {
   AGIMP_SCENE
-  const LevelDesc& Ll = T.lv[l];
+  const LevelDesc Ll = T.lv[l];
   size_t tot = 1;
   for (int a = 0; a < D; ++a) tot *= (size_t)Ll.e;
   size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -1433,7 +1491,7 @@ __global__ void k_dense_level(const SceneDesc* __restrict__ Dsc, int l, int* cel
 template <int D>
 __global__ void k_dense_node(const SceneDesc* __restrict__ Dsc, int* out) {
   AGIMP_SCENE
-  const LevelDesc& Ll = T.lv[T.L - 1];
+  const LevelDesc Ll = T.lv[T.L - 1];
   size_t tot = 1;
   for (int a = 0; a < D; ++a) tot *= (size_t)Ll.e;
   size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;

@@ -102,8 +102,8 @@ template <int D>
 __device__ __forceinline__ void record_requests(const Topo& T, int ulev, const float* x, int g, int* dst,
                                                 int stride, const int* near = nullptr) {
   if (g < ulev || g >= T.L) return;
-  const LevelDesc& Lg = T.lv[g];
-  const LevelDesc& Lu = T.lv[ulev];
+  const LevelDesc Lg = T.lv[g];
+  const LevelDesc Lu = T.lv[ulev];
   int b[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Lg, T.pow2);
@@ -192,7 +192,7 @@ __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, Le
   if (blockIdx.x * blockDim.x >= (unsigned)n) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = k < n;
-  const LevelDesc& Lu = T.lv[U.ulev];
+  const LevelDesc Lu = T.lv[U.ulev];
   int u = -1, g = -1;
   float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
   float m = 0.f, vn = 0.f, cn = 0.f, jm1 = 0.f, njac = 0.f;
@@ -327,7 +327,7 @@ __global__ void k_unit_compact(const uint8_t* __restrict__ flags, int n, int* li
 
 template <int D>
 __global__ void k_unit_dilate(const SceneDesc* __restrict__ Dsc, LearnDesc U) {
-  const LevelDesc& Lu = Dsc[0].T.lv[U.ulev];
+  const LevelDesc Lu = Dsc[0].T.lv[U.ulev];
   const int no = U.counts[0];
   constexpr int NN = (D == 2) ? 9 : 27;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)no * NN;
@@ -353,7 +353,7 @@ __device__ __forceinline__ float lev_code(int lp1, int L) {
 template <int D>
 __global__ void k_unit_features(const SceneDesc* __restrict__ Dsc, LearnDesc U) {
   const Topo& T = Dsc[0].T;
-  const LevelDesc& Lu = T.lv[U.ulev];
+  const LevelDesc Lu = T.lv[U.ulev];
   const int L = T.L;
   const int nc = U.counts[1];
   const float disp = U.win * Lu.inv_dx;  // velocity -> displacement in units over the window
@@ -484,7 +484,7 @@ __global__ void k_activate_ahead(const SceneDesc* __restrict__ Dsc, int par, Lea
   if (k >= sd.n) return;
   const int g = P.g[k];
   if (g < 0 || g >= T.L) return;
-  const LevelDesc& Lg = T.lv[g];
+  const LevelDesc Lg = T.lv[g];
   int lo[3], ext[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) {
@@ -517,7 +517,7 @@ __global__ void k_activate_ahead(const SceneDesc* __restrict__ Dsc, int par, Lea
 template <int D>
 __global__ void k_unit_activate(const SceneDesc* __restrict__ Dsc, LearnDesc U) {
   const Topo& T = Dsc[0].T;
-  const LevelDesc& Lu = T.lv[U.ulev];
+  const LevelDesc Lu = T.lv[U.ulev];
   const int nc = U.counts[1];
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < (int64_t)nc * P4<D>::v;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -535,7 +535,7 @@ __global__ void k_unit_activate(const SceneDesc* __restrict__ Dsc, LearnDesc U) 
     unit_coords<D>(Lu, q, J);
 #pragma unroll
     for (int a = 0; a < D; ++a) I[a] = (J[a] - PAD) * side + off[a];
-    const LevelDesc& Ll = T.lv[lev];
+    const LevelDesc Ll = T.lv[lev];
     int Jl[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) Jl[a] = I[a] + PAD;
