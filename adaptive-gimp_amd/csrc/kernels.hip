@@ -921,6 +921,9 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
     // 2. contributions, reduced per cell
     const float scale = -ph.dt * Ll.inv_dx * ph.p_vol;  // -dt (0.5/r)^d / dx * V_p with r = 0.5
     for (int base = 0; base < ck.z; base += P2G_T) {
+      // a wavefront whose lanes all lie past the chunk's end skips the pass (uniform per
+      // wavefront; the pass has no barrier)
+      if (base + (tid & ~63) >= ck.z) continue;
       const int i = base + tid;
       const bool valid = i < ck.z;
       const int k = valid ? sidx[i] : 0;

@@ -22,12 +22,25 @@ namespace agimp {
 
 struct Mat3d { double m[3][3]; };
 
+// 1 / sqrt(x) in FP64 (the device library's reciprocal square root: one operation instead
+// of a square root and a division)
+AGIMP_HD inline double rsqrt_d(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return ::rsqrt(x);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
+
 AGIMP_HD inline void jacobi_rotate(double A[3][3], double V[3][3], int p, int q) {
   double apq = A[p][q];
   if (fabs(apq) < 1e-300) return;
-  double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
-  double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-  double c = 1.0 / sqrt(t * t + 1.0);
+  // t = tan of the rotation angle, the root of t^2 + 2 theta t - 1 = 0 with |t| <= 1,
+  // theta = (aqq - app) / (2 apq); with d = aqq - app and e = 2 apq the same root is
+  // t = sgn(d) e / (|d| + sqrt(d^2 + e^2)), one division instead of three
+  const double d = A[q][q] - A[p][p], e = 2.0 * apq;
+  double t = (d >= 0.0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
+  double c = rsqrt_d(t * t + 1.0);
   double s = t * c;
   // A' = J^T A J with J the rotation in the (p,q) plane
   int r = 3 - p - q;
@@ -49,9 +62,13 @@ AGIMP_HD inline void jacobi_rotate(double A[3][3], double V[3][3], int p, int q)
 // Givens rotation that zeroes B[j][k] against B[i][k] acting on rows i, j; accumulates U.
 AGIMP_HD inline void givens_rows(double B[3][3], double U[3][3], int i, int j, int k) {
   double a = B[i][k], b = B[j][k];
-  double rr = sqrt(a * a + b * b);
+  double r2 = a * a + b * b;
   double c = 1.0, s = 0.0;
-  if (rr > 1e-300) { c = a / rr; s = b / rr; }
+  if (r2 > 1e-300) {
+    const double inv = rsqrt_d(r2);
+    c = a * inv;
+    s = b * inv;
+  }
 #pragma unroll
   for (int col = 0; col < 3; ++col) {
     double bi = B[i][col], bj = B[j][col];
