@@ -512,8 +512,8 @@ __global__ void k_activate_ahead(const SceneDesc* __restrict__ Dsc, int par, Lea
   }
 }
 
-// cells of the predicted level inside every refined unit get ACTIVATED (between the
-// particle activation and the culling of the rebuild)
+// cells of the predicted level inside every refined unit without particles get ACTIVATED
+// (between the particle activation and the culling of the rebuild)
 template <int D>
 __global__ void k_unit_activate(const SceneDesc* __restrict__ Dsc, LearnDesc U) {
   const Topo& T = Dsc[0].T;
@@ -525,6 +525,10 @@ __global__ void k_unit_activate(const SceneDesc* __restrict__ Dsc, LearnDesc U) 
     const int w = (int)(t % P4<D>::v);
     const int lev = U.upred[q];
     if (lev < U.ulev) continue;
+    // only units without particles now: requests inside occupied units come from their own
+    // particles (current and swept requests), and finer cells over them would move those
+    // particles' transfer level fl_p away from the plain method's
+    if (U.uflag[q]) continue;
     const int s = lev - U.ulev;  // 0..2: 2^s cells per axis
     const int side = 1 << s;
     int r = w, off[3];
