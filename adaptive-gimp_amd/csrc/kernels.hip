@@ -543,6 +543,8 @@ __device__ __forceinline__ bool unit_covers(const Topo& T, int ulev, const int8_
 }
 
 // ------------------------------------------------------------------------------ bucketing
+constexpr int LM_TABLE_BITS = 8, LM_TABLE = 1 << LM_TABLE_BITS;  // >= threads per workgroup
+constexpr int LM_EMPTY = -2;
 // adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
 // particle's 2^d cells; rank inside its (level, leaf block) bucket by run-aggregated atomics.
 template <int D>
@@ -613,11 +615,46 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
       }
     }
   }
-  int* addr = (gbid >= 0) ? &bcnt[gbid] : frozen;
-  int r = wave_run_atomic_add(addr, live);
+  // rank inside the block: runs of equal blocks inside each wavefront are merged per
+  // workgroup in an LDS table (local offsets from LDS atomics), then one global atomic per
+  // (workgroup, block) reserves the workgroup's range: consecutive wavefronts mostly fill the
+  // same block, and one returning global atomic per run made them queue on its counter
+  __shared__ int tkey[LM_TABLE], tcnt[LM_TABLE], tbase[LM_TABLE];
+  for (int i = threadIdx.x; i < LM_TABLE; i += blockDim.x) {
+    tkey[i] = LM_EMPTY;
+    tcnt[i] = 0;
+  }
+  __syncthreads();
+  const int lane = __lane_id();
+  const int key = live ? gbid : LM_EMPTY;  // -1: frozen
+  const int prev = __shfl_up(key, 1);
+  const bool head = !live || lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+  const int h = 63 - __clzll(heads & le);
+  const unsigned long long above = heads & ~((h == 63) ? ~0ull : ((1ull << (h + 1)) - 1ull));
+  const int nh = above ? (__ffsll((long long)above) - 1) : 64;
+  int slot = 0, loc = 0;
+  if (live && lane == h) {
+    const uint32_t hs = ((uint32_t)(key + 1) * 2654435761u) >> (32 - LM_TABLE_BITS);
+    for (int probe = 0; probe < LM_TABLE; ++probe) {
+      const int e = (int)((hs + probe) & (LM_TABLE - 1));
+      const int old = atomicCAS(&tkey[e], LM_EMPTY, key);
+      if (old == LM_EMPTY || old == key) { slot = e; break; }
+    }
+    loc = atomicAdd(&tcnt[slot], nh - h);
+  }
+  slot = __shfl(slot, h);
+  loc = __shfl(loc, h);
+  __syncthreads();
+  for (int i = threadIdx.x; i < LM_TABLE; i += blockDim.x) {
+    const int kk = tkey[i];
+    if (kk != LM_EMPTY) tbase[i] = atomicAdd(kk >= 0 ? &bcnt[kk] : frozen, tcnt[i]);
+  }
+  __syncthreads();
   if (live) {
     gb[k] = gbid;
-    rk[k] = r;
+    rk[k] = tbase[slot] + loc + (lane - h);
   }
 }
 
