@@ -252,4 +252,26 @@ __device__ __forceinline__ int wave_run_atomic_add(int* addr, bool active) {
   return base + (lane - h);
 }
 
+// Slot reservation for scattered active lanes (compactions): lanes are grouped by address,
+// one atomic per distinct address in the wavefront (one per counter, however the active
+// lanes interleave), slots in lane order.  The loop is uniform over the wavefront.
+__device__ __forceinline__ int wave_agg_atomic_add(int* addr, bool active) {
+  const int lane = __lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long a_bits = (unsigned long long)(uintptr_t)addr;
+  unsigned long long todo = __ballot(active);
+  int slot = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const unsigned long long la = __shfl(a_bits, leader);
+    const unsigned long long grp = __ballot(active && a_bits == la);
+    int base = 0;
+    if (lane == leader) base = atomicAdd(addr, __popcll(grp));
+    base = __shfl(base, leader);
+    if ((grp >> lane) & 1ull) slot = base + __popcll(grp & below);
+    todo &= ~grp;
+  }
+  return slot;
+}
+
 }  // namespace agimp

@@ -138,15 +138,6 @@ __global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc) {
   fill_zero((uint32_t*)T.ccount, 4);
 }
 
-// before bucketing: block counters, the frozen counter, grid-phase list counters
-__global__ void k_clear_bucket(const SceneDesc* __restrict__ Dsc) {
-  AGIMP_SCENE
-  fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
-  fill_zero((uint32_t*)sd.S.frozen, 1);
-  fill_zero((uint32_t*)T.scount, MAXL);
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
-}
-
 // the valid-node block flags of the octree: save after refine (to_saved) or restore
 __global__ void k_copy_flags(const SceneDesc* __restrict__ Dsc, int to_saved) {
   AGIMP_SCENE
@@ -168,7 +159,7 @@ __global__ void k_compact_cells(const SceneDesc* __restrict__ Dsc, int which) {
   int* count = T.ccount + which;
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   bool on = g < n && flags[g] != 0;
-  int idx = wave_run_atomic_add(count, on);
+  int idx = wave_agg_atomic_add(count, on);
   if (on) list[idx] = g;
 }
 
@@ -192,6 +183,37 @@ __global__ void k_clear_sgrid(const SceneDesc* __restrict__ Dsc) {
       acc += cnt;
     }
   }
+}
+
+// before bucketing, in one launch: zero the grid of the previous grid-phase lists (read
+// here, so the list counters are zeroed later, by the bucketing pass), the block counters
+// and the frozen counter; restore the octree's valid-node block flags; count the substep
+template <int D>
+__global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc) {
+  AGIMP_SCENE
+  size_t total = 0;
+  for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  for (size_t t = t0; t < total; t += st) {
+    size_t acc = 0;
+    for (int l = 0; l < T.L; ++l) {
+      size_t cnt = (size_t)T.scount[l] * Pow4<D>::v;
+      if (t < acc + cnt) {
+        size_t q = t - acc;
+        int b = T.lv[l].slist[q / Pow4<D>::v];
+        T.lv[l].grid[(size_t)b * Pow4<D>::v + q % Pow4<D>::v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        break;
+      }
+      acc += cnt;
+    }
+  }
+  fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
+  fill_zero((uint32_t*)sd.S.frozen, 1);
+  uint32_t* a = (uint32_t*)T.lv[0].sflag;
+  const uint32_t* v = sd.vflag;
+  for (size_t i = t0; i < sd.w_flags; i += st) a[i] = v[i];
+  if (t0 == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
 }
 
 // ------------------------------------------------------------------------------ activation
@@ -557,6 +579,8 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
   int* bcnt = sd.S.bcnt;
   int* frozen = sd.S.frozen;
   unsigned long long* cnt = sd.cnt;
+  // the grid-phase list counters (the prep pass read them; k_scompact appends from zero)
+  if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
   if (blockIdx.x * blockDim.x >= (unsigned)n) return;  // whole workgroup past this scene
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = k < n;
@@ -742,46 +766,38 @@ __global__ void k_scan_final(const SceneDesc* __restrict__ Dsc) {
   }
 }
 
-__global__ void k_scatter(const SceneDesc* __restrict__ Dsc) {
+// after the scan, in one launch: the sorted position of every particle (threads t < n) and,
+// for every occupied leaf block (threads t < total_blocks), its P2G chunk descriptors and the
+// blocks its 3^d stencils touch
+template <int D>
+__global__ void k_scatter_blocks(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int n = sd.n;
-  const int* gb = sd.S.gb;
-  const int* rk = sd.S.rk;
   const int* boff = sd.S.boff;
   const int* totals = sd.S.totals;
-  int* perm = sd.S.perm;
-  unsigned long long* cnt = sd.cnt;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int g = gb[k];
-  int pos = (g >= 0) ? boff[g] + rk[k] : totals[0] + rk[k];
-  perm[pos] = k;
-  if (k == 0) {
-    cnt[CNT_ACTIVE] += (unsigned long long)totals[0];
-    cnt[CNT_FROZEN] += (unsigned long long)(n - totals[0]);
-    cnt[CNT_CHUNKS] += (unsigned long long)totals[1];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const int g = sd.S.gb[t];
+    const int r = sd.S.rk[t];
+    const int pos = (g >= 0) ? boff[g] + r : totals[0] + r;
+    sd.S.perm[pos] = t;
+    if (t == 0) {
+      unsigned long long* cnt = sd.cnt;
+      cnt[CNT_ACTIVE] += (unsigned long long)totals[0];
+      cnt[CNT_FROZEN] += (unsigned long long)(n - totals[0]);
+      cnt[CNT_CHUNKS] += (unsigned long long)totals[1];
+    }
   }
-}
-
-// per occupied leaf block: P2G chunk descriptors + the blocks its 3^d stencils touch
-template <int D>
-__global__ void k_blocks(const SceneDesc* __restrict__ Dsc) {
-  AGIMP_SCENE
-  const int* bcnt = sd.S.bcnt;
-  const int* boff = sd.S.boff;
-  const int* choff = sd.S.choff;
-  int4* chunks = sd.S.chunks;
-  const int CH = sd.CH;
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= T.total_blocks) return;
-  int c = bcnt[g];
+  if (t >= T.total_blocks) return;
+  const int c = sd.S.bcnt[t];
   if (c == 0) return;
-  int l = level_of_gbid(T, g);
+  const int CH = sd.CH;
+  const int l = level_of_gbid(T, t);
   const LevelDesc Ll = T.lv[l];
-  int b = g - Ll.gbase;
-  int nch = (c + CH - 1) / CH;
-  int o = choff[g], s = boff[g];
-  for (int j = 0; j < nch; ++j) chunks[o + j] = make_int4(g, s + j * CH, min(CH, c - j * CH), 0);
+  const int b = t - Ll.gbase;
+  const int nch = (c + CH - 1) / CH;
+  const int o = sd.S.choff[t], s0 = boff[t];
+  for (int j = 0; j < nch; ++j) sd.S.chunks[o + j] = make_int4(t, s0 + j * CH, min(CH, c - j * CH), 0);
   int B[3];
   block_coords<D>(Ll.NB, b, B);
 #pragma unroll
@@ -810,7 +826,7 @@ __global__ void k_scompact(const SceneDesc* __restrict__ Dsc) {
     b = g - T.lv[l].gbase;
     on = T.lv[l].sflag[b] != 0;
   }
-  int idx = wave_run_atomic_add(&T.scount[l], on);
+  int idx = wave_agg_atomic_add(&T.scount[l], on);
   if (on) T.lv[l].slist[idx] = b;
 }
 
@@ -1594,16 +1610,14 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
   const dim3 lg = grid2(m.list_grid, m.nscenes);
   const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
+  const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {
-    hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_clear_bucket, fg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 0);
+    hipLaunchKernelGGL(k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
     hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_tiles, grid2(1, m.nscenes), dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);
-    hipLaunchKernelGGL(k_scatter, pg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_blocks<DD>, bg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_scompact<DD>, bg, dim3(256), 0, s, d);
   });
 }

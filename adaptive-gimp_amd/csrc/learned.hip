@@ -321,7 +321,7 @@ __global__ void k_unit_requests(const SceneDesc* __restrict__ Dsc, int par, Lear
 __global__ void k_unit_compact(const uint8_t* __restrict__ flags, int n, int* list, int* count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool on = i < n && flags[i] != 0;
-  const int idx = wave_run_atomic_add(count, on);
+  const int idx = wave_agg_atomic_add(count, on);
   if (on) list[idx] = i;
 }
 
