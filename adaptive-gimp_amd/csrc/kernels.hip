@@ -417,14 +417,19 @@ __device__ __forceinline__ int face_parent(const Topo& T, int l, const int* I, i
 #pragma unroll
   for (int a = 0; a < D; ++a) inr = inr && (I1[a] >= 0) && (I1[a] < T.lv[l].n);
   if (!inr) return -1;
-  // _heriarchical_check :155-158
+  // _heriarchical_check :155-158 (unrolled over the level bound: the cell loads of all
+  // levels are independent and issue together)
   int lp = -1;
-  for (int lq = 0; lq <= l; ++lq) {
-    int P[3];
 #pragma unroll
-    for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - lq);
-    int64_t qq = lidx<D>(T.lv[lq], P);
-    if (qq >= 0 && T.lv[lq].cell[qq] == ACTIVATED) lp = lq;
+  for (int lq = 0; lq < MAXL; ++lq) {
+    if (lq <= l) {
+      int P[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - lq);
+      const LevelDesc Lq = T.lv[lq];
+      int64_t qq = lidx<D>(Lq, P);
+      if (qq >= 0 && Lq.cell[qq] == ACTIVATED) lp = lq;
+    }
   }
   if (lp == l) return -1;  // cell_mask[l][I1] == ACTIVATED
   return lp;
