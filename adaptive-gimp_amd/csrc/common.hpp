@@ -252,6 +252,14 @@ __device__ __forceinline__ int wave_run_atomic_add(int* addr, bool active) {
   return base + (lane - h);
 }
 
+// LDS written by some lanes of a wavefront and read by others: order the accesses and keep
+// the compiler from moving them across this point (the wavefront itself runs in lockstep)
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Slot reservation for scattered active lanes (compactions): lanes are grouped by address,
 // one atomic per distinct address in the wavefront (one per counter, however the active
 // lanes interleave), slots in lane order.  The loop is uniform over the wavefront.

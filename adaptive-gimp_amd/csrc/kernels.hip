@@ -1048,23 +1048,40 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
             }
           }
       } else {
+        // p_a = mw (vb_a + C_a . i) + S_a . grad w, with mw = w0 w1 w2 m and grad w =
+        // (dw0 w1 w2, w0 dw1 w2, w0 w1 dw2), regrouped so that most of it is computed once per
+        // (i0, i1):  p_a = w2 F_a + w01 E_a(i2) + mw i2 C_a2, where
+        // F_a = w01 m (vb_a + C_a0 i0 + C_a1 i1) + S_a0 dw0 w1 + S_a1 w0 dw1, E_a(i2) = S_a2 dw2
+        float E[3][3], C2[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          C2[a] = 2.0f * Cm[a][2];
+#pragma unroll
+          for (int i2 = 0; i2 < 3; ++i2) E[a][i2] = S[a][2] * dw[2][i2];
+        }
 #pragma unroll
         for (int i0 = 0; i0 < 3; ++i0)
 #pragma unroll
-          for (int i1 = 0; i1 < 3; ++i1)
+          for (int i1 = 0; i1 < 3; ++i1) {
+            const float w01 = w[0][i0] * w[1][i1];
+            const float m01 = w01 * mp;
+            const float h0 = dw[0][i0] * w[1][i1], h1 = w[0][i0] * dw[1][i1];
+            float F[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+              F[a] = m01 * (vb[a] + Cm[a][0] * i0 + Cm[a][1] * i1) + (S[a][0] * h0 + S[a][1] * h1);
 #pragma unroll
             for (int i2 = 0; i2 < 3; ++i2) {
-              float w01 = w[0][i0] * w[1][i1];
-              float wt = w01 * w[2][i2];
-              float g0 = dw[0][i0] * w[1][i1] * w[2][i2];
-              float g1 = w[0][i0] * dw[1][i1] * w[2][i2];
-              float g2 = w01 * dw[2][i2];
-              float mw = wt * mp;
+              const float w2 = w[2][i2];
+              float mw = m01 * w2;
               float p[3];
 #pragma unroll
-              for (int a = 0; a < 3; ++a)
-                p[a] = mw * (vb[a] + Cm[a][0] * i0 + Cm[a][1] * i1 + Cm[a][2] * i2) + S[a][0] * g0 + S[a][1] * g1 +
-                       S[a][2] * g2;
+              for (int a = 0; a < 3; ++a) {
+                float pa = w2 * F[a] + w01 * E[a][i2];
+                if (i2 == 1) pa += mw * Cm[a][2];
+                if (i2 == 2) pa += mw * C2[a];
+                p[a] = pa;
+              }
               p[0] = seg_sum(p[0], sm);
               p[1] = seg_sum(p[1], sm);
               p[2] = seg_sum(p[2], sm);
@@ -1077,6 +1094,7 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
                 atomicAdd(&tile[nd + 3], mw);
               }
             }
+          }
       }
     }
     __syncthreads();

@@ -7,6 +7,7 @@
 #include <cmath>
 
 #include "agimp.h"
+#include "common.hpp"
 #include "mlp.hpp"
 
 namespace agimp {
@@ -47,12 +48,6 @@ __device__ __forceinline__ void stash(uint16_t* S, int mb, bf16x4 v) {
   const int n = lane & 15, g = lane >> 4;
 #pragma unroll
   for (int j = 0; j < 4; ++j) S[(mb * 16 + 4 * g + j) * 16 + n] = (uint16_t)v[j];
-}
-
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Mean (weighted) softmax cross-entropy and its gradient with respect to the parameters:
