@@ -730,44 +730,39 @@ __global__ void k_scan_reduce(const SceneDesc* __restrict__ Dsc) {
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_tiles(const SceneDesc* __restrict__ Dsc) {
-  AGIMP_SCENE
-  int2* tile_sum = sd.S.tiles;
-  const int ntiles = (T.total_blocks + SCAN_TILE - 1) / SCAN_TILE;
-  int* totals = sd.S.totals;
-  __shared__ int2 sh[SCAN_T];
-  int2 carry = make_int2(0, 0);
-  for (int base = 0; base < ntiles; base += SCAN_T) {
-    int i = base + threadIdx.x;
-    int2 v = (i < ntiles) ? tile_sum[i] : make_int2(0, 0);
-    int2 tot;
-    int2 ex = block_excl_scan(v, sh, tot);
-    if (i < ntiles) tile_sum[i] = add2(ex, carry);
-    carry = add2(carry, tot);
-  }
-  if (threadIdx.x == 0) { totals[0] = carry.x; totals[1] = carry.y; }
-}
-
+// Final pass: each tile's offset is the sum of the totals of the tiles before it (a few
+// hundred at most, read by this workgroup), so no separate pass scans the tile totals; the
+// last tile also writes the grand totals (active particles, chunks).
 __global__ void k_scan_final(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* bcnt = sd.S.bcnt;
   const int n = T.total_blocks, CH = sd.CH;
-  const int2* tile_off = sd.S.tiles;
+  const int2* tile_sum = sd.S.tiles;
   int* boff = sd.S.boff;
   int* choff = sd.S.choff;
-  if ((int)blockIdx.x * SCAN_TILE >= n) return;
+  const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if ((int)blockIdx.x >= ntiles) return;
   __shared__ int2 sh[SCAN_T];
+  int2 o = make_int2(0, 0);
+  for (int i = threadIdx.x; i < (int)blockIdx.x; i += SCAN_T) o = add2(o, tile_sum[i]);
+  int2 tile_off;
+  block_excl_scan(o, sh, tile_off);  // tile_off = the workgroup's total of o
   int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
   int2 v[SCAN_I];
   int2 s = make_int2(0, 0);
 #pragma unroll
   for (int i = 0; i < SCAN_I; ++i) { v[i] = load_cc(bcnt, base + i, n, CH); s = add2(s, v[i]); }
   int2 tot;
-  int2 run = add2(block_excl_scan(s, sh, tot), tile_off[blockIdx.x]);
+  int2 run = add2(block_excl_scan(s, sh, tot), tile_off);
 #pragma unroll
   for (int i = 0; i < SCAN_I; ++i) {
     if (base + i < n) { boff[base + i] = run.x; choff[base + i] = run.y; }
     run = add2(run, v[i]);
+  }
+  if ((int)blockIdx.x == ntiles - 1 && threadIdx.x == 0) {
+    const int2 all = add2(tile_off, tot);
+    sd.S.totals[0] = all.x;
+    sd.S.totals[1] = all.y;
   }
 }
 
@@ -1638,7 +1633,6 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
     hipLaunchKernelGGL(k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
     hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
-    hipLaunchKernelGGL(k_scan_tiles, grid2(1, m.nscenes), dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_scompact<DD>, bg, dim3(256), 0, s, d);
