@@ -1165,60 +1165,79 @@ __device__ __forceinline__ bool in_range_nodes(const int* I, int n) {
   return ok;
 }
 
-// adaptive_gimp.py:289-301 as a gather onto each valid coarse node I of level l-1:
-// sources are fine nodes 2I - d with weight 2^-|d|_1.
+// adaptive_gimp.py:289-301 as a gather onto each valid coarse node I of level l-1: sources
+// are the fine nodes 2I - d, d in {-1,0,1}^d, with weight 2^-|d|_1.  Whether a fine node
+// contributes, and its weight (d_a != 0 exactly where the fine coordinate is odd), depend on
+// the fine node alone, so one wavefront per coarse leaf block first stages the weighted
+// contributions of the 9^d fine nodes around the block in LDS (each fine node loaded once
+// instead of once per coarse neighbour), then every lane sums its coarse node's 3^d entries
+// in the same order as a direct gather.
 template <int D>
-__global__ void k_restrict(const SceneDesc* __restrict__ Dsc, int l) {
+__global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ Dsc, int l) {
   AGIMP_SCENE
   if (l >= T.L) return;
   const LevelDesc Lc = T.lv[l - 1];
   const LevelDesc Lf = T.lv[l];
   const NodeView N = node_view(T);
-  int total = T.scount[l - 1] * Pow4<D>::v;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    int I[3];
-    uint32_t q = list_node<D>(Lc, t / Pow4<D>::v, t % Pow4<D>::v, I);
-    if (!in_range_nodes<D>(I, Lc.n)) continue;
-    if (!f_valid(nflag<D>(N, Lc.shift, I))) continue;
-    // the node flags of all sources first, then their values: independent loads in flight
-    // together instead of one flag -> value round trip per source
-    constexpr int NS = (D == 2) ? 9 : 27;
-    int qf[NS];
-    uint8_t M[NS];
+  constexpr int NW = 4, R = 9, RN = (D == 2) ? R * R : R * R * R;
+  __shared__ float4 tile[NW][RN];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nblk = T.scount[l - 1];
+  for (int e = blockIdx.x * NW + wv; e < nblk; e += gridDim.x * NW) {  // uniform per wavefront
+    const int b = Lc.slist[e];
+    int B[3], f0[3];
+    block_coords<D>(Lc.NB, b, B);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      int If[3], r = s;
+    for (int a = 0; a < D; ++a) f0[a] = 2 * (4 * B[a] - PAD) - 1;  // first fine node (unpadded)
+    for (int i = lane; i < RN; i += 64) {
+      int If[3], r = i, nodd = 0;
 #pragma unroll
-      for (int a = 0; a < D; ++a) { If[a] = 2 * I[a] - ((r % 3) - 1); r /= 3; }
-      qf[s] = -1;
-      M[s] = 0;
+      for (int a = D - 1; a >= 0; --a) { If[a] = f0[a] + r % R; r /= R; nodd += If[a] & 1; }
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       if (in_range_nodes<D>(If, Lf.n)) {
-        const int64_t qq = lidx<D>(Lf, If);
-        if (qq >= 0) {
-          qf[s] = (int)qq;
-          M[s] = nflag<D>(N, Lf.shift, If);
+        const int64_t qf = lidx<D>(Lf, If);
+        if (qf >= 0) {
+          const uint8_t M = nflag<D>(N, Lf.shift, If);
+          if ((nodd == 0 && f_act(M)) || f_T(M) || f_ghost(M)) {
+            const float4 g = Lf.grid[qf];
+            const float wgt = (nodd == 0) ? 1.f : (nodd == 1 ? 0.5f : (nodd == 2 ? 0.25f : 0.125f));
+            v = make_float4(g.x * wgt, g.y * wgt, g.z * wgt, g.w * wgt);
+          }
+        }
+      }
+      tile[wv][i] = v;
+    }
+    wave_sync_lds();
+    if (lane < Pow4<D>::v) {
+      int I[3], w[3];
+      if (D == 2) { w[0] = lane >> 2; w[1] = lane & 3; }
+      else { w[0] = lane >> 4; w[1] = (lane >> 2) & 3; w[2] = lane & 3; }
+#pragma unroll
+      for (int a = 0; a < D; ++a) I[a] = 4 * B[a] - PAD + w[a];
+      if (in_range_nodes<D>(I, Lc.n) && f_valid(nflag<D>(N, Lc.shift, I))) {
+        constexpr int NS = (D == 2) ? 9 : 27;
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2) {
+          int r = s2, idx = 0;
+          int rr[3];
+#pragma unroll
+          for (int a = 0; a < D; ++a) { rr[a] = 2 * w[a] + 1 - ((r % 3) - 1); r /= 3; }  // If - f0
+#pragma unroll
+          for (int a = 0; a < D; ++a) idx = idx * R + rr[a];
+          const float4 v = tile[wv][idx];
+          acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+        }
+        // no source contributed: the node keeps its value (adding zeros changes nothing)
+        if (acc.x != 0.f || acc.y != 0.f || acc.z != 0.f || acc.w != 0.f) {
+          const uint32_t q = (uint32_t)b * Pow4<D>::v + lane;
+          float4 g = Lc.grid[q];
+          g.x += acc.x; g.y += acc.y; g.z += acc.z; g.w += acc.w;
+          Lc.grid[q] = g;
         }
       }
     }
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    bool any = false;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      int r = s, l1 = 0;
-#pragma unroll
-      for (int a = 0; a < D; ++a) { l1 += (r % 3) != 1; r /= 3; }
-      const bool cond = qf[s] >= 0 && ((l1 == 0 && f_act(M[s])) || f_T(M[s]) || f_ghost(M[s]));
-      if (cond) {
-        const float4 val = Lf.grid[qf[s]];
-        const float wgt = (l1 == 0) ? 1.f : (l1 == 1 ? 0.5f : (l1 == 2 ? 0.25f : 0.125f));
-        acc.x += val.x * wgt; acc.y += val.y * wgt; acc.z += val.z * wgt; acc.w += val.w * wgt;
-        any = true;
-      }
-    }
-    if (!any) continue;
-    float4 g = Lc.grid[q];
-    g.x += acc.x; g.y += acc.y; g.z += acc.z; g.w += acc.w;
-    Lc.grid[q] = g;
+    wave_sync_lds();  // the tile is refilled for the next block
   }
 }
 
