@@ -108,9 +108,9 @@ def _new_sim(pkg, wl, ppc):
                             particles_per_chunk=ppc, **wl.params)
 
 
-def _learned_sim(pkg, wl, ppc, model, parts, tau=0.3):
+def _learned_sim(pkg, wl, ppc, model, parts, tau=0.3, window=0):
     sim = _new_sim(pkg, wl, ppc)
-    sim.learned_enable(model[1].get("window", 20), tau)
+    sim.learned_enable(window or model[1].get("window", 20), tau)
     sim.learned_set_model(model[0])
     sim.learned_set_parts(network="network" in parts, sweep="sweep" in parts)
     return sim
@@ -120,7 +120,7 @@ def bench_learned(pkg, wl, args, np, model, plain_x):
     """The same workload with predicted octrees (agimp_learned_advance): throughput over the
     same warmup + steps from the same initial state, rebuild statistics, and the positions'
     deviation from the plain run at the end."""
-    sim = _learned_sim(pkg, wl, args.ppc, model, args.learned_parts, args.tau)
+    sim = _learned_sim(pkg, wl, args.ppc, model, args.learned_parts, args.tau, args.window)
     sim.advance_learned(wl.dt, args.warmup)
     sim.sync()
     sim.learned_reset_stats()
@@ -155,7 +155,7 @@ def accuracy(pkg, wl, args, np, model):
             for _ in range(n):
                 sim.substep(wl.dt)
         else:
-            sim = _learned_sim(pkg, wl, args.ppc, mdl, args.learned_parts, args.tau)
+            sim = _learned_sim(pkg, wl, args.ppc, mdl, args.learned_parts, args.tau, args.window)
             sim.advance_learned(wl.dt, n)
         x = sim.x_p.to_numpy().astype(np.float64)
         v = sim.v_p.to_numpy().astype(np.float64)
@@ -245,6 +245,8 @@ def main():
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"],
                     help="single-scene workloads: time the plain path, the learned path (predicted octrees, "
                          "adaptive-gimp_amd/refine_mlp.npz), or both (value = the plain path)")
+    ap.add_argument("--window", type=int, default=0,
+                    help="learned path: substeps per predicted octree (0: the model's training window)")
     ap.add_argument("--tau", type=float, default=0.3,
                     help="learned path: decision threshold on the cumulative class probability")
     ap.add_argument("--learned-parts", default="network,sweep",
@@ -327,7 +329,7 @@ def main():
             lv = total_n * args.steps / el
             line["learned"] = {
                 "value": lv, "ms_per_step": el * 1e3 / args.steps, "speedup_vs_plain": lv / line["value"],
-                "parts": args.learned_parts, "window": model[1].get("window", 20), "tau": args.tau,
+                "parts": args.learned_parts, "window": args.window or model[1].get("window", 20), "tau": args.tau,
                 "stats": st, "criterion_failures_executed": 0,
                 "criterion_failures_detected": st["violations"],
                 "x_dev_from_plain": {"max": float(dev.max()), "p99": float(np.percentile(dev, 99)),
