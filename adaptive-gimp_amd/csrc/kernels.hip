@@ -1449,8 +1449,8 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
       Fd[i][j] = (double)acc;
     }
   double Jp = (double)Pin.f[(size_t)PL::JP * Pin.stride + k];
-  double tau[D][D];
-  constitutive<D>(Fd, Jp, ph, tau, true);
+  // everything that does not depend on the SVD is stored first: shorter live ranges across
+  // the FP64 SVD, and the stores drain while it runs
   bool finite = true;
 #pragma unroll
   for (int a = 0; a < D; ++a) {
@@ -1462,20 +1462,23 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-      Pout.f[(size_t)(PL::C + i * D + j) * Pout.stride + pos] = G[i][j];
-      Pout.f[(size_t)(PL::F + i * D + j) * Pout.stride + pos] = (float)Fd[i][j];
-    }
-#pragma unroll
-  for (int i = 0; i < D; ++i)
-#pragma unroll
-    for (int j = i; j < D; ++j) Pout.f[(size_t)(PL::TAU + tau_idx<D>(i, j)) * Pout.stride + pos] = (float)tau[i][j];
+    for (int j = 0; j < D; ++j) Pout.f[(size_t)(PL::C + i * D + j) * Pout.stride + pos] = G[i][j];
   Pout.f[(size_t)PL::M * Pout.stride + pos] = Pin.f[(size_t)PL::M * Pin.stride + k];
-  Pout.f[(size_t)PL::JP * Pout.stride + pos] = (float)Jp;
   Pout.g[pos] = Pin.g[k];
   Pout.id[pos] = Pin.id[k];
   flv[pos] = l;
   if (!finite) count(cnt, CNT_NAN);
+  double tau[D][D];
+  constitutive<D>(Fd, Jp, ph, tau, true);
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = 0; j < D; ++j) Pout.f[(size_t)(PL::F + i * D + j) * Pout.stride + pos] = (float)Fd[i][j];
+#pragma unroll
+  for (int i = 0; i < D; ++i)
+#pragma unroll
+    for (int j = i; j < D; ++j) Pout.f[(size_t)(PL::TAU + tau_idx<D>(i, j)) * Pout.stride + pos] = (float)tau[i][j];
+  Pout.f[(size_t)PL::JP * Pout.stride + pos] = (float)Jp;
 }
 
 // Kirchhoff stress of the stored F (no plastic projection): after the caller sets state
