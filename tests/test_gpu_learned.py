@@ -264,3 +264,29 @@ def test_swept_requests_cover_straight_motion(pkg):
     sim.close()
     assert st["forced"] == 0 and st["rebuilds"] == 5
     assert np.abs(xl - xp).max() < 1e-6
+
+
+def test_trained_network_prevents_rollbacks(pkg, learned):
+    """What the trained network gains over the method without it: on the small falling
+    sphere, 100 substeps in windows of 20 with the predicted octree made of the current
+    requests plus the network's units (no swept requests).  A model that predicts nothing
+    (class 3 everywhere) leaves every boundary crossing to the criterion check, which cuts
+    most windows short and rolls them back; the trained network predicts the refinement the
+    window needs, so no window is cut short.  Both runs end at the same substep count."""
+    wl = pkg.scenes.small_3d(seed=0)
+    params, meta = learned.load_model()
+    stats = {}
+    for name, p in (("none", _const_model(3)), ("trained", params)):
+        sim = pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(), None, **wl.params)
+        sim.learned_enable(meta["window"], 0.3)
+        sim.learned_set_model(p)
+        sim.learned_set_parts(network=True, sweep=False)
+        sim.advance_learned(wl.dt, 100)
+        stats[name] = sim.learned_stats()
+        c = sim.counters()
+        sim.close()
+        assert stats[name]["substeps"] == 100 and c["nonfinite"] == 0
+    print(stats)
+    assert stats["none"]["forced"] >= 5 and stats["none"]["discarded"] >= 50
+    assert stats["trained"]["forced"] <= 1 and stats["trained"]["refined"] > 0
+    assert stats["trained"]["discarded"] * 4 < stats["none"]["discarded"]
