@@ -193,6 +193,7 @@ Dims single_dims(const agimp_handle* h) {
   m.dim = h->D;
   m.nscenes = 1;
   m.maxL = h->L;
+  m.any_single = h->L == 1;
   m.any_static = h->hd.static_mode;
   m.any_dynamic = !h->hd.static_mode;
   m.max_n = (size_t)h->n;
@@ -977,6 +978,7 @@ int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out
     (void)hipStreamSynchronize(h->stream);
     b->scenes.push_back(h);
     m.maxL = std::max(m.maxL, h->L);
+    m.any_single |= h->L == 1;
     m.any_static |= h->hd.static_mode;
     m.any_dynamic |= !h->hd.static_mode;
     m.max_n = std::max(m.max_n, (size_t)h->n);

@@ -1173,9 +1173,11 @@ __device__ __forceinline__ bool in_range_nodes(const int* I, int n) {
 // instead of once per coarse neighbour), then every lane sums its coarse node's 3^d entries
 // in the same order as a direct gather.
 template <int D>
-__global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ Dsc, int l) {
+__global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ Dsc, int l, float dt_arg) {
   AGIMP_SCENE
   if (l >= T.L) return;
+  Physics ph = sd.ph;
+  if (dt_arg > 0.f) ph.dt = dt_arg;
   const LevelDesc Lc = T.lv[l - 1];
   const LevelDesc Lf = T.lv[l];
   const NodeView N = node_view(T);
@@ -1214,7 +1216,8 @@ __global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ 
       else { w[0] = lane >> 4; w[1] = (lane >> 2) & 3; w[2] = lane & 3; }
 #pragma unroll
       for (int a = 0; a < D; ++a) I[a] = 4 * B[a] - PAD + w[a];
-      if (in_range_nodes<D>(I, Lc.n) && f_valid(nflag<D>(N, Lc.shift, I))) {
+      const uint8_t Mc = in_range_nodes<D>(I, Lc.n) ? nflag<D>(N, Lc.shift, I) : (uint8_t)0;
+      if (f_valid(Mc)) {
         constexpr int NS = (D == 2) ? 9 : 27;
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -1228,12 +1231,15 @@ __global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ 
           const float4 v = tile[wv][idx];
           acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
         }
-        // no source contributed: the node keeps its value (adding zeros changes nothing)
-        if (acc.x != 0.f || acc.y != 0.f || acc.z != 0.f || acc.w != 0.f) {
+        // no source contributed: the node keeps its value (adding zeros changes nothing);
+        // level 0 is complete after this restriction, so its grid update is applied here
+        const bool add = acc.x != 0.f || acc.y != 0.f || acc.z != 0.f || acc.w != 0.f;
+        if (add || l == 1) {
           const uint32_t q = (uint32_t)b * Pow4<D>::v + lane;
           float4 g = Lc.grid[q];
           g.x += acc.x; g.y += acc.y; g.z += acc.z; g.w += acc.w;
-          Lc.grid[q] = g;
+          const bool upd = (l == 1) && grid_update<D>(T, Lc, ph, I, Mc, g);
+          if (add || upd) Lc.grid[q] = g;
         }
       }
     }
@@ -1241,63 +1247,67 @@ __global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ 
   }
 }
 
-// adaptive_gimp.py:277-287 on every level's list (+ sticky walls / slab collider)
+// adaptive_gimp.py:277-287 for node I of level l, whose value is g (momentum, mass):
+// momentum -> velocity, gravity, walls (+ sticky walls / slab collider).  Only active nodes
+// with mass change; returns whether g changed.  Levels >= 1 exclude the nodes prolongation
+// overwrites (T, ghost, all-even): their callers never pass those.
+template <int D>
+__device__ __forceinline__ bool grid_update(const Topo& T, const LevelDesc& Ll, const Physics& ph, const int* I,
+                                            uint8_t M, float4& g) {
+  if (!(g.w > 0.f && f_act(M))) return false;
+  float v[3] = {g.x, g.y, g.z};
+#pragma unroll
+  for (int a = 0; a < D; ++a) { v[a] /= g.w; v[a] += ph.gravity[a] * ph.dt; }
+  bool wall_any = false;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    int f = I[a] << Ll.shift;
+    bool wl = f < ph.boundary_gap || f > T.finest_size - ph.boundary_gap;
+    wall_any = wall_any || wl;
+    if (wl && ph.wall_mode == 0) v[a] = 0.f;
+  }
+  if (ph.wall_mode == 1 && wall_any)
+#pragma unroll
+    for (int a = 0; a < D; ++a) v[a] = 0.f;
+  if (ph.slab_enable) {
+    double y = (double)I[ph.slab_axis] * (double)Ll.dx;
+    if (y >= (double)ph.slab_lo && y <= (double)ph.slab_hi)
+#pragma unroll
+      for (int a = 0; a < D; ++a) v[a] = 0.f;
+  }
+  g.x = v[0]; g.y = v[1];
+  if (D == 3) g.z = v[2];
+  return true;
+}
+
+// The grid update of single-level scenes.  With two or more levels it is fused into the
+// last restriction (level 0) and into each prolongation (levels >= 1).
 template <int D>
 __global__ void k_gridop(const SceneDesc* __restrict__ Dsc, float dt_arg) {
   AGIMP_SCENE
+  if (T.L >= 2) return;
   Physics ph = sd.ph;
   if (dt_arg > 0.f) ph.dt = dt_arg;
-  int tot[MAXL];
-  int total = 0;
-  for (int l = 0; l < T.L; ++l) { tot[l] = T.scount[l] * Pow4<D>::v; total += tot[l]; }
+  const LevelDesc L0 = T.lv[0];
+  const NodeView N = node_view(T);
+  const int total = T.scount[0] * Pow4<D>::v;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    int l = 0, r = t;
-    while (r >= tot[l]) { r -= tot[l]; ++l; }
-    const LevelDesc Ll = T.lv[l];
     int I[3];
-    uint32_t q = list_node<D>(Ll, r / Pow4<D>::v, r % Pow4<D>::v, I);
-    if (!in_range_nodes<D>(I, Ll.n)) continue;
-    uint8_t M = nflag<D>(T, l, I);
-    if (l >= 1) {
-      bool even = true;
-#pragma unroll
-      for (int a = 0; a < D; ++a) even = even && ((I[a] & 1) == 0);
-      if (f_T(M) || f_ghost(M) || even) continue;  // overwritten by prolongation
-    }
-    float4 g = Ll.grid[q];
-    if (!(g.w > 0.f && f_act(M))) continue;
-    float v[3] = {g.x, g.y, g.z};
-#pragma unroll
-    for (int a = 0; a < D; ++a) { v[a] /= g.w; v[a] += ph.gravity[a] * ph.dt; }
-    bool wall_any = false;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      int f = I[a] << Ll.shift;
-      bool wl = f < ph.boundary_gap || f > T.finest_size - ph.boundary_gap;
-      wall_any = wall_any || wl;
-      if (wl && ph.wall_mode == 0) v[a] = 0.f;
-    }
-    if (ph.wall_mode == 1 && wall_any)
-#pragma unroll
-      for (int a = 0; a < D; ++a) v[a] = 0.f;
-    if (ph.slab_enable) {
-      double y = (double)I[ph.slab_axis] * (double)Ll.dx;
-      if (y >= (double)ph.slab_lo && y <= (double)ph.slab_hi)
-#pragma unroll
-        for (int a = 0; a < D; ++a) v[a] = 0.f;
-    }
-    g.x = v[0]; g.y = v[1];
-    if (D == 3) g.z = v[2];
-    Ll.grid[q] = g;
+    uint32_t q = list_node<D>(L0, t / Pow4<D>::v, t % Pow4<D>::v, I);
+    if (!in_range_nodes<D>(I, L0.n)) continue;
+    float4 g = L0.grid[q];
+    if (grid_update<D>(T, L0, ph, I, nflag<D>(N, L0.shift, I), g)) L0.grid[q] = g;
   }
 }
 
 // adaptive_gimp.py:303-321 as a gather onto each level-l node J that is T, ghost or
 // all-even: J = 2I + d over valid coarse nodes I, weight 2^-|d|_1 (only d = 0 unless T/G).
 template <int D>
-__global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l) {
+__global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l, float dt_arg) {
   AGIMP_SCENE
   if (l >= T.L) return;
+  Physics ph = sd.ph;
+  if (dt_arg > 0.f) ph.dt = dt_arg;
   const LevelDesc Lf = T.lv[l];
   const LevelDesc Lc = T.lv[l - 1];
   const NodeView N = node_view(T);
@@ -1311,7 +1321,11 @@ __global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l) {
     bool even = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) even = even && ((J[a] & 1) == 0);
-    if (!(isT || isG || even)) continue;
+    if (!(isT || isG || even)) {  // not overwritten by prolongation: level l's grid update
+      float4 g = Lf.grid[q];
+      if (grid_update<D>(T, Lf, ph, J, M, g)) Lf.grid[q] = g;
+      continue;
+    }
     float acc[3] = {0.f, 0.f, 0.f};
     if (isT || isG || isA) {
       int nodd = 0;
@@ -1668,9 +1682,9 @@ void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int 
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
   const dim3 gg = grid2(m.grid_grid, m.nscenes);
   AGIMP_DIM(m.dim, {
-    for (int l = m.maxL - 1; l >= 1; --l) hipLaunchKernelGGL(k_restrict<DD>, gg, dim3(256), 0, s, d, l);
-    hipLaunchKernelGGL(k_gridop<DD>, gg, dim3(256), 0, s, d, dt);
-    for (int l = 1; l < m.maxL; ++l) hipLaunchKernelGGL(k_prolong<DD>, gg, dim3(256), 0, s, d, l);
+    for (int l = m.maxL - 1; l >= 1; --l) hipLaunchKernelGGL(k_restrict<DD>, gg, dim3(256), 0, s, d, l, dt);
+    if (m.any_single) hipLaunchKernelGGL(k_gridop<DD>, gg, dim3(256), 0, s, d, dt);
+    for (int l = 1; l < m.maxL; ++l) hipLaunchKernelGGL(k_prolong<DD>, gg, dim3(256), 0, s, d, l, dt);
   });
 }
 

@@ -11,6 +11,7 @@ struct Dims {
   int dim;
   int nscenes;
   int maxL;
+  int any_single;           // some scene has one level (its grid update is not fused)
   int any_static, any_dynamic;
   size_t max_n;             // largest particle count
   size_t max_total_blocks;  // largest number of leaf blocks over all levels
