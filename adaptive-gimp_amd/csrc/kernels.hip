@@ -1165,6 +1165,39 @@ __device__ __forceinline__ bool in_range_nodes(const int* I, int n) {
   return ok;
 }
 
+// adaptive_gimp.py:277-287 for node I of level l, whose value is g (momentum, mass):
+// momentum -> velocity, gravity, walls (+ sticky walls / slab collider).  Only active nodes
+// with mass change; returns whether g changed.  Levels >= 1 exclude the nodes prolongation
+// overwrites (T, ghost, all-even): their callers never pass those.
+template <int D>
+__device__ __forceinline__ bool grid_update(const Topo& T, const LevelDesc& Ll, const Physics& ph, const int* I,
+                                            uint8_t M, float4& g) {
+  if (!(g.w > 0.f && f_act(M))) return false;
+  float v[3] = {g.x, g.y, g.z};
+#pragma unroll
+  for (int a = 0; a < D; ++a) { v[a] /= g.w; v[a] += ph.gravity[a] * ph.dt; }
+  bool wall_any = false;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    int f = I[a] << Ll.shift;
+    bool wl = f < ph.boundary_gap || f > T.finest_size - ph.boundary_gap;
+    wall_any = wall_any || wl;
+    if (wl && ph.wall_mode == 0) v[a] = 0.f;
+  }
+  if (ph.wall_mode == 1 && wall_any)
+#pragma unroll
+    for (int a = 0; a < D; ++a) v[a] = 0.f;
+  if (ph.slab_enable) {
+    double y = (double)I[ph.slab_axis] * (double)Ll.dx;
+    if (y >= (double)ph.slab_lo && y <= (double)ph.slab_hi)
+#pragma unroll
+      for (int a = 0; a < D; ++a) v[a] = 0.f;
+  }
+  g.x = v[0]; g.y = v[1];
+  if (D == 3) g.z = v[2];
+  return true;
+}
+
 // adaptive_gimp.py:289-301 as a gather onto each valid coarse node I of level l-1: sources
 // are the fine nodes 2I - d, d in {-1,0,1}^d, with weight 2^-|d|_1.  Whether a fine node
 // contributes, and its weight (d_a != 0 exactly where the fine coordinate is odd), depend on
@@ -1245,39 +1278,6 @@ __global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ 
     }
     wave_sync_lds();  // the tile is refilled for the next block
   }
-}
-
-// adaptive_gimp.py:277-287 for node I of level l, whose value is g (momentum, mass):
-// momentum -> velocity, gravity, walls (+ sticky walls / slab collider).  Only active nodes
-// with mass change; returns whether g changed.  Levels >= 1 exclude the nodes prolongation
-// overwrites (T, ghost, all-even): their callers never pass those.
-template <int D>
-__device__ __forceinline__ bool grid_update(const Topo& T, const LevelDesc& Ll, const Physics& ph, const int* I,
-                                            uint8_t M, float4& g) {
-  if (!(g.w > 0.f && f_act(M))) return false;
-  float v[3] = {g.x, g.y, g.z};
-#pragma unroll
-  for (int a = 0; a < D; ++a) { v[a] /= g.w; v[a] += ph.gravity[a] * ph.dt; }
-  bool wall_any = false;
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    int f = I[a] << Ll.shift;
-    bool wl = f < ph.boundary_gap || f > T.finest_size - ph.boundary_gap;
-    wall_any = wall_any || wl;
-    if (wl && ph.wall_mode == 0) v[a] = 0.f;
-  }
-  if (ph.wall_mode == 1 && wall_any)
-#pragma unroll
-    for (int a = 0; a < D; ++a) v[a] = 0.f;
-  if (ph.slab_enable) {
-    double y = (double)I[ph.slab_axis] * (double)Ll.dx;
-    if (y >= (double)ph.slab_lo && y <= (double)ph.slab_hi)
-#pragma unroll
-      for (int a = 0; a < D; ++a) v[a] = 0.f;
-  }
-  g.x = v[0]; g.y = v[1];
-  if (D == 3) g.z = v[2];
-  return true;
 }
 
 // The grid update of single-level scenes.  With two or more levels it is fused into the
