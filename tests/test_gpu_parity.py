@@ -281,3 +281,27 @@ def test_headline_scene_full_size_properties(pkg, O):
     o2.refine()
     assert_topology_equal(sim, o2, O, wl.level)
     sim.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_single_level_scene(pkg, O, dim):
+    """level = 1 (the reference's MPM special case): no restriction or prolongation, so the
+    grid update runs as its own pass; grid and particles track the oracle after one substep,
+    particles over 5."""
+    rng = np.random.default_rng(11 + dim)
+    n = 3000
+    x = rng.uniform(0.3, 0.7, (n, dim))
+    st = dict(x=x, v=0.5 * rng.standard_normal((n, dim)), F=np.tile(np.eye(dim), (n, 1, 1)),
+              C=np.zeros((n, dim, dim)), m=np.ones(n), g=np.zeros(n, np.int32))
+    sim, ora = make_pair(pkg, O, dim, 1, 32, st, {})
+    sim.substep(1e-4)
+    ora.substep(1e-4)
+    assert_topology_equal(sim, ora, O, 1)
+    assert_grid_close(sim, ora, 1)
+    assert_particles_close(particle_errors(sim, ora))
+    for _ in range(4):
+        sim.substep(1e-4)
+        ora.substep(1e-4)
+    e = particle_errors(sim, ora)  # FP32 rounding drift over 5 substeps: the multi-substep bars
+    assert e["fl"] == 0 and e["x"] <= 1e-5 and e["v"] <= 1e-3 and e["F"] <= 1e-4, e
+    sim.close()
