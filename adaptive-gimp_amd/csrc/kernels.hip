@@ -471,8 +471,9 @@ __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
       if (!any) continue;
       int64_t qq = lidx<D>(T.lv[T.L - 1], f);
       if (qq < 0) { count(cnt, CNT_OOB); continue; }
+      // a T node lies on the face this cell shares with I1, so it is a corner of this
+      // activated cell: k_node_A has already flagged its grid-phase blocks
       T.node[qq] |= T_JUNCTION;
-      mark_valid_node<D>(T, f);
     }
   }
   }
