@@ -104,20 +104,29 @@ __device__ __forceinline__ void count(unsigned long long* cnt, int which, unsign
 // mark the grid-phase blocks of every level on which finest node f is a node
 template <int D>
 __device__ __forceinline__ void mark_valid_node(const Topo& T, const int* f) {
-  for (int l = T.L - 1; l >= 0; --l) {
-    int s = T.lv[l].shift;
-    int msk = (1 << s) - 1;
-    bool div = true;
+  // unrolled over the level bound: the level descriptors are read once per kernel, not per
+  // node and level (from the finest level down; a node stops being one at the first level
+  // whose spacing does not divide it, and stays so at all coarser ones)
+  bool alive = true;
 #pragma unroll
-    for (int a = 0; a < D; ++a) div = div && ((f[a] & msk) == 0);
-    if (!div) break;
-    int J[3];
+  for (int i = 0; i < MAXL; ++i) {
+    const int l = T.L - 1 - i;
+    if (l >= 0) {
+      const LevelDesc Lv = T.lv[l];
+      const int msk = (1 << Lv.shift) - 1;
+      bool div = true;
 #pragma unroll
-    for (int a = 0; a < D; ++a) J[a] = (f[a] >> s) + PAD;
-    if (!in_box<D>(T.lv[l].E, J)) continue;
-    int b = (D == 2) ? ((J[0] >> 2) * T.lv[l].NB + (J[1] >> 2))
-                     : (((J[0] >> 2) * T.lv[l].NB + (J[1] >> 2)) * T.lv[l].NB + (J[2] >> 2));
-    T.lv[l].sflag[b] = 1;
+      for (int a = 0; a < D; ++a) div = div && ((f[a] & msk) == 0);
+      alive = alive && div;
+      int J[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) J[a] = (f[a] >> Lv.shift) + PAD;
+      if (alive && in_box<D>(Lv.E, J)) {
+        const int b = (D == 2) ? ((J[0] >> 2) * Lv.NB + (J[1] >> 2))
+                               : (((J[0] >> 2) * Lv.NB + (J[1] >> 2)) * Lv.NB + (J[2] >> 2));
+        Lv.sflag[b] = 1;
+      }
+    }
   }
 }
 
