@@ -583,7 +583,7 @@ __device__ __forceinline__ bool unit_covers(const Topo& T, int ulev, const int8_
 constexpr int LM_TABLE_BITS = 8, LM_TABLE = 1 << LM_TABLE_BITS;  // >= threads per workgroup
 constexpr int LM_EMPTY = -2;
 // adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
-// particle's 2^d cells; rank inside its (level, leaf block) bucket by run-aggregated atomics.
+// particle's 2^d cells; rank inside its (level, leaf block) bucket (run sums merged per workgroup).
 template <int D>
 __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
   AGIMP_SCENE
