@@ -3,8 +3,8 @@
 The compute path is libagimp.so (HIP kernels for gfx950, C-ABI in include/agimp.h); this
 package is the Python host side mirroring ``adaptive_gimp.py``'s ``AdaptiveGIMP`` class.
 """
-from . import scenes
+from . import render, scenes
 from ._lib import AgimpError
 from .sim import ACTIVATED, CULLING, GHOST, T_JUNCTION, UNACTIVATED, AdaptiveGIMP, Batch
 
-__all__ = ["AdaptiveGIMP", "Batch", "AgimpError", "scenes", "UNACTIVATED", "ACTIVATED", "GHOST", "T_JUNCTION", "CULLING"]
+__all__ = ["AdaptiveGIMP", "Batch", "AgimpError", "scenes", "render", "UNACTIVATED", "ACTIVATED", "GHOST", "T_JUNCTION", "CULLING"]

@@ -106,6 +106,11 @@ class AdaptiveGIMP:
                  particles_per_chunk=1024, device=0):
         if dim not in (2, 3):
             raise ValueError("dim must be 2 or 3")
+        self._kwargs = dict(E=E, nu=nu, gravity=None if gravity is None else [float(v) for v in gravity],
+                            p_mass=p_mass, p_vol=p_vol, boundary_gap=boundary_gap, wall_mode=wall_mode,
+                            plastic=bool(plastic), theta_c=theta_c, theta_s=theta_s, hardening=hardening,
+                            slab=None if slab is None else [int(slab[0]), float(slab[1]), float(slab[2])],
+                            particles_per_chunk=particles_per_chunk)
         self.dim = dim
         self.level = level
         self.coarsest_size = coarsest_size
@@ -287,6 +292,62 @@ class AdaptiveGIMP:
         _lib.check(self._L.agimp_get_counters(self._h, _lib.ptr(out, ctypes.c_int64), 8), self._h, "get_counters")
         return {"oob": int(out[0]), "frozen": int(out[1]), "active": int(out[2]), "chunks": int(out[3]),
                 "nonfinite": int(out[4]), "criterion_failures": int(out[5])}
+
+    def profile_report(self):
+        """Per-phase GPU time since the last call, as text: the counterpart of the Taichi kernel
+        profiler report that run_3d.py:12 and :43 print.  Phases are timed with HIP events on
+        the library's stream while ``set_timing(True)`` is on."""
+        t = self.timings()
+        lines = [f"{'phase':<10} {'calls':>7} {'total ms':>11} {'avg ms':>9}"]
+        for name, (ms, calls) in t.items():
+            if calls:
+                lines.append(f"{name:<10} {calls:>7d} {ms:>11.3f} {ms / calls:>9.4f}")
+        tot = sum(ms for ms, _ in t.values())
+        lines.append(f"{'total':<10} {'':>7} {tot:>11.3f}")
+        return "\n".join(lines)
+
+    # ------------------------------------------------------------------ snapshots (not in the reference)
+    def save(self, path, **meta):
+        """Checkpoint to an ``.npz`` file: the particle state (x, v, F, C, m, colour, g, J_p, in
+        the caller's particle order) and the constructor settings, as plain arrays and JSON (no
+        pickled objects).  ``meta`` (JSON-serialisable) is stored alongside, e.g. the time."""
+        import json
+
+        self._pull()
+        H = self._host
+        cfg = dict(self._kwargs, dim=self.dim, level=self.level, coarsest_size=self.coarsest_size,
+                   n_particles=self.n_particles, static_adaptivity=bool(self.static_adaptivity))
+        np.savez(path, x=H["x_p"], v=H["v_p"], F=H["F_p"], C=H["C_p"], m=H["m_p"], c=H["c_p"], g=H["g_p"],
+                 Jp=H["Jp_p"], config=np.array(json.dumps(cfg)), meta=np.array(json.dumps(meta)))
+
+    @classmethod
+    def load(cls, path, grid_initializer=None, **overrides):
+        """A simulator resumed from a checkpoint written by ``save`` (read with
+        ``allow_pickle=False``).  A static-adaptivity run passes its ``grid_initializer`` again
+        (a callable is not stored); ``overrides`` replace stored constructor settings.
+        Returns ``(sim, meta)``."""
+        import json
+
+        with np.load(path, allow_pickle=False) as z:
+            cfg = json.loads(str(z["config"]))
+            meta = json.loads(str(z["meta"]))
+            st = {k: z[k] for k in ("x", "v", "F", "C", "m", "c", "g", "Jp")}
+        if cfg.pop("static_adaptivity") and grid_initializer is None:
+            raise ValueError("checkpoint of a static-adaptivity run: pass its grid_initializer")
+        dim, level, c, n = (cfg.pop(k) for k in ("dim", "level", "coarsest_size", "n_particles"))
+        cfg.update(overrides)
+
+        def init(sim):
+            sim.x_p.from_numpy(st["x"])
+            sim.v_p.from_numpy(st["v"])
+            sim.F_p.from_numpy(st["F"])
+            sim.C_p.from_numpy(st["C"])
+            sim.m_p.from_numpy(st["m"])
+            sim.c_p.from_numpy(st["c"])
+            sim.g_p.from_numpy(st["g"])
+            sim.Jp_p.from_numpy(st["Jp"])
+
+        return cls(dim, level, c, n, init, grid_initializer, **cfg), meta
 
     # ------------------------------------------------------------------ learned prediction
     def learned_enable(self, window=20, tau=0.3):

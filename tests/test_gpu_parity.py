@@ -305,3 +305,76 @@ def test_single_level_scene(pkg, O, dim):
     e = particle_errors(sim, ora)  # FP32 rounding drift over 5 substeps: the multi-substep bars
     assert e["fl"] == 0 and e["x"] <= 1e-5 and e["v"] <= 1e-3 and e["F"] <= 1e-4, e
     sim.close()
+
+
+def test_snapshot_resume(pkg, O, tmp_path):
+    """save() -> load() restores the particle state exactly and the constructor settings
+    (plasticity, walls, material); the resumed run's octree is bit-exact with the original's
+    and both runs track each other over further substeps (P2G sums in another particle order,
+    so FP32 rounding, not state, separates them)."""
+    dim, L, c, st, prm, gi, dt = scene(pkg, "small3d_plastic")
+    a, _ = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    for _ in range(3):
+        a.substep(dt)
+    path = tmp_path / "snap.npz"
+    a.save(path, t=3 * dt, note="three substeps")
+    b, meta = pkg.AdaptiveGIMP.load(path)
+    assert meta == {"t": 3 * dt, "note": "three substeps"}
+    assert b._kwargs == a._kwargs and (b.dim, b.level, b.coarsest_size, b.n_particles) == (dim, L, c, a.n_particles)
+    for k in ("x_p", "v_p", "F_p", "C_p", "m_p", "c_p", "g_p", "Jp_p"):
+        assert np.array_equal(getattr(a, k).to_numpy(), getattr(b, k).to_numpy()), k
+    a.refine()
+    b.refine()
+    for l in range(L):
+        assert np.array_equal(a.cell_mask[l].to_numpy(), b.cell_mask[l].to_numpy()), l
+    assert np.array_equal(O.canonical_node_mask(a.node_mask.to_numpy()), O.canonical_node_mask(b.node_mask.to_numpy()))
+    for _ in range(3):
+        a.substep(dt)
+        b.substep(dt)
+    ga = {k: getattr(a, k).to_numpy() for k in ("x_p", "v_p", "F_p", "Jp_p", "fl_p")}
+    gb = {k: getattr(b, k).to_numpy() for k in ("x_p", "v_p", "F_p", "Jp_p", "fl_p")}
+    vs = np.abs(ga["v_p"]).max()
+    assert np.array_equal(ga["fl_p"], gb["fl_p"])
+    assert np.abs(ga["x_p"] - gb["x_p"]).max() <= 1e-5
+    assert np.abs(ga["v_p"] - gb["v_p"]).max() <= 1e-3 * vs
+    assert np.abs(ga["F_p"] - gb["F_p"]).max() <= 1e-4 and np.abs(ga["Jp_p"] - gb["Jp_p"]).max() <= 1e-4
+    a.close()
+    b.close()
+
+
+def test_profile_report_lists_phases(pkg):
+    wl = pkg.scenes.small_3d(seed=2, n=4000, L=3, c=16)
+    st = wl.state
+    sim = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, st["x"].shape[0], _init_from(st), None, **wl.params)
+    sim.set_timing(True)
+    for _ in range(2):
+        sim.substep(wl.dt)
+    rep = sim.profile_report()
+    rows = {ln.split()[0]: ln.split() for ln in rep.splitlines()[1:]}
+    assert "total" in rows and float(rows["total"][-1]) > 0
+    assert any(r[0] != "total" and int(r[1]) >= 2 for r in rows.values()), rep
+    sim.close()
+
+
+@pytest.mark.parametrize("name", ["ref2d_dynamic", "small3d_plastic"])
+def test_render_matches_oracle_octree(pkg, O, name, tmp_path):
+    """The PNG of the GPU simulator's octree equals, pixel for pixel, the picture of the
+    oracle's octree for the same particles (2D whole domain; 3D the slice z = 0.5)."""
+    dim, L, c, st, prm, gi, dt = scene(pkg, name)
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    sim.refine()
+    ora.refine()
+    R = pkg.render
+    res = 4 * c * 2 ** (L - 1)
+    p = tmp_path / "octree.png"
+    got = R.render(sim, p, res=res, show_particles=False)
+    cells = [ora.get_level(l)[0] for l in range(L)]
+    nodes = ora.get_node_mask()
+    if dim == 3:
+        cells, nodes = R.slice_masks(cells, nodes, c)
+    want = R.to_rgb8(R.mask_image(cells, nodes, c, res))
+    assert np.array_equal(got, want)
+    assert np.array_equal(R.read_png(p), got)
+    with_p = R.render(sim, res=res)
+    assert (with_p != got).any(axis=2).sum() > 0
+    sim.close()
