@@ -444,7 +444,9 @@ __device__ __forceinline__ int face_parent(const Topo& T, int l, const int* I, i
   return lp;
 }
 
-// adaptive_gimp.py:170-174 for all levels at once
+// adaptive_gimp.py:170-174 for all levels at once.  The coarse-fine faces found here are also
+// recorded, as one word per listed cell (3 bits per face: activated parent level + 1, or 0),
+// so that the ghost marks need not search again.
 template <int D>
 __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
@@ -455,12 +457,13 @@ __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
   uint32_t q;
-  if (!decode_list<D>(T, list, nlist, t, l, I, q)) continue;
-  if (T.lv[l].cell[q] != ACTIVATED) continue;
+  uint32_t word = 0;
+  if (decode_list<D>(T, list, nlist, t, l, I, q) && T.lv[l].cell[q] == ACTIVATED) {
   for (int j = 0; j < 2 * D; ++j) {
     int I1[3];
     int lp = face_parent<D>(T, l, I, j, I1);
     if (lp < 0) continue;
+    word |= (uint32_t)(lp + 1) << (3 * j);
     int ax = j >> 1, side = j & 1;
     int msk = (1 << (l - lp)) - 1;
 #pragma unroll
@@ -486,9 +489,37 @@ __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
     }
   }
   }
+  T.fword[t] = word;
+  }
 }
 
-// adaptive_gimp.py:176-183 for all levels at once, after every T mark
+// ghost marks of one coarse-fine face (neighbour I1 of level l, activated parent level lp)
+template <int D>
+__device__ __forceinline__ void mark_G_face(const Topo& T, unsigned long long* cnt, const int* I1, int l, int lp) {
+  for (int l0 = lp + 1; l0 <= l; ++l0) {
+    int P[3];
+#pragma unroll
+    for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - l0);
+    int64_t qc = lidx<D>(T.lv[l0], P);
+    if (qc < 0) { count(cnt, CNT_OOB); continue; }
+    T.lv[l0].cell[qc] |= GHOST;
+#pragma unroll
+    for (int u = 0; u < (1 << D); ++u) {
+      int f[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) f[a] = (P[a] + ((u >> (D - 1 - a)) & 1)) << T.lv[l0].shift;
+      int64_t qq = lidx<D>(T.lv[T.L - 1], f);
+      if (qq < 0) { count(cnt, CNT_OOB); continue; }
+      if (T.node[qq] == 0) {
+        T.node[qq] = GHOST;
+        mark_valid_node<D>(T, f);
+      }
+    }
+  }
+}
+
+// adaptive_gimp.py:176-183 for all levels at once, after every T mark, over the coarse-fine
+// faces the T marks recorded
 template <int D>
 __global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
@@ -496,36 +527,54 @@ __global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
   const int* countp = T.ccount + 2;
   unsigned long long* cnt = sd.cnt;
   const int nlist = *countp;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
-  int l, I[3];
-  uint32_t q;
-  if (!decode_list<D>(T, list, nlist, t, l, I, q)) continue;
-  if (T.lv[l].cell[q] != ACTIVATED) continue;
-  for (int j = 0; j < 2 * D; ++j) {
-    int I1[3];
-    int lp = face_parent<D>(T, l, I, j, I1);
-    if (lp < 0) continue;
-    for (int l0 = lp + 1; l0 <= l; ++l0) {
-      int P[3];
+  // Each wavefront loads the words of 64 consecutive listed cells, then spreads their recorded
+  // faces over its lanes, 64 faces per round (lane f takes the f-th face in cell order).  The
+  // loop is uniform over each workgroup, so every lane takes part in the shuffles.
+  const int N = nlist * Pow4<D>::v;
+  const int lane = __lane_id();
+  for (int base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
+    const int t = base + threadIdx.x;
+    const uint32_t word = t < N ? T.fword[t] : 0u;
+    int nface = 0;
 #pragma unroll
-      for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - l0);
-      int64_t qc = lidx<D>(T.lv[l0], P);
-      if (qc < 0) { count(cnt, CNT_OOB); continue; }
-      T.lv[l0].cell[qc] |= GHOST;
+    for (int j = 0; j < 2 * D; ++j) nface += ((word >> (3 * j)) & 7u) != 0u;
+    int incl = nface;
 #pragma unroll
-      for (int u = 0; u < (1 << D); ++u) {
-        int f[3];
-#pragma unroll
-        for (int a = 0; a < D; ++a) f[a] = (P[a] + ((u >> (D - 1 - a)) & 1)) << T.lv[l0].shift;
-        int64_t qq = lidx<D>(T.lv[T.L - 1], f);
-        if (qq < 0) { count(cnt, CNT_OOB); continue; }
-        if (T.node[qq] == 0) {
-          T.node[qq] = GHOST;
-          mark_valid_node<D>(T, f);
-        }
-      }
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
     }
-  }
+    const int total = __shfl(incl, 63);
+    const int t0 = t - lane;  // the wavefront's first cell
+    for (int f0 = 0; f0 < total; f0 += 64) {
+      const int f = f0 + lane;
+      // source lane: the first with incl > f
+      int src = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1) {
+        const int probe = __shfl(incl, src + step - 1);
+        if (probe <= f) src += step;
+      }
+      const uint32_t w = __shfl(word, src);
+      const int k = f - (__shfl(incl, src) - __shfl(nface, src));  // rank of the face in its cell
+      if (f >= total) continue;
+      int j = 0, seen = 0;
+#pragma unroll
+      for (int jj = 0; jj < 2 * D; ++jj) {
+        const bool on = ((w >> (3 * jj)) & 7u) != 0u;
+        if (on && seen == k) j = jj;
+        seen += on;
+      }
+      const int lp = (int)((w >> (3 * j)) & 7u) - 1;
+      int l, I[3];
+      uint32_t q;
+      if (!decode_list<D>(T, list, nlist, t0 + src, l, I, q)) continue;
+      int I1[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) I1[a] = I[a];
+      I1[j >> 1] += (j & 1) ? -1 : 1;
+      mark_G_face<D>(T, cnt, I1, l, lp);
+    }
   }
 }
 
