@@ -459,6 +459,7 @@ __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
   uint32_t q;
   uint32_t word = 0;
   if (decode_list<D>(T, list, nlist, t, l, I, q) && T.lv[l].cell[q] == ACTIVATED) {
+#pragma unroll
   for (int j = 0; j < 2 * D; ++j) {
     int I1[3];
     int lp = face_parent<D>(T, l, I, j, I1);
