@@ -188,6 +188,16 @@ struct LearnDesc {
 };
 
 // block-major index of padded coordinates J (each in [0, E))
+// bidx is a sum of per-axis terms: bidx(NB, J) = sum_a bidx_axis<D>(NB, a, J[a]), so a
+// stencil's addresses need the 3 terms of each axis, not a full index per node
+template <int D>
+__device__ __forceinline__ uint32_t bidx_axis(int NB, int a, int j) {
+  const uint32_t nb = (uint32_t)NB;
+  const uint32_t S = (D == 2) ? (a == 0 ? nb * 16u : 16u) : (a == 0 ? nb * nb * 64u : (a == 1 ? nb * 64u : 64u));
+  const uint32_t s = (D == 2) ? (a == 0 ? 4u : 1u) : (a == 0 ? 16u : (a == 1 ? 4u : 1u));
+  return (uint32_t)(j >> 2) * S + (uint32_t)(j & 3) * s;
+}
+
 template <int D>
 __device__ __forceinline__ uint32_t bidx(int NB, const int* J) {
   if (D == 2) {

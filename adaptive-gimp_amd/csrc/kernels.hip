@@ -1470,14 +1470,18 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
     bspline(qx - (float)(b + 1), w[a], dw[a]);
     J[a] = b + PAD;
   }
+  uint32_t ca[3][3];  // per-axis address terms of the stencil nodes
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ca[a][i] = bidx_axis<D>(Ll.NB, a, J[a] + i);
   float nv[3] = {0.f, 0.f, 0.f}, G[3][3] = {};
   if constexpr (D == 2) {
 #pragma unroll
     for (int i0 = 0; i0 < 3; ++i0)
 #pragma unroll
       for (int i1 = 0; i1 < 3; ++i1) {
-        int Jn[2] = {J[0] + i0, J[1] + i1};
-        float4 gv = Ll.grid[bidx<2>(Ll.NB, Jn)];
+        float4 gv = Ll.grid[ca[0][i0] + ca[1][i1]];
         float wt = w[0][i0] * w[1][i1];
         float g0 = dw[0][i0] * w[1][i1], g1 = w[0][i0] * dw[1][i1];
         nv[0] += gv.x * wt; nv[1] += gv.y * wt;
@@ -1491,8 +1495,7 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
       for (int i1 = 0; i1 < 3; ++i1)
 #pragma unroll
         for (int i2 = 0; i2 < 3; ++i2) {
-          int Jn[3] = {J[0] + i0, J[1] + i1, J[2] + i2};
-          float4 gv = Ll.grid[bidx<3>(Ll.NB, Jn)];
+          float4 gv = Ll.grid[ca[0][i0] + ca[1][i1] + ca[2][i2]];
           float w01 = w[0][i0] * w[1][i1];
           float wt = w01 * w[2][i2];
           float g0 = dw[0][i0] * w[1][i1] * w[2][i2];
