@@ -159,17 +159,47 @@ __global__ void k_copy_flags(const SceneDesc* __restrict__ Dsc, int to_saved) {
   }
 }
 
-// append the ids of set flags to list (order unspecified)
-__global__ void k_compact_cells(const SceneDesc* __restrict__ Dsc, int which) {
+constexpr int COMPACT_T = 256;  // threads per workgroup of k_compact_cells (4 flags each)
+
+// append the ids of set flags to list: each thread reads one word of 4 flags, a workgroup
+// reserves its slots with one atomic, and ids are written in increasing order within it
+__global__ void __launch_bounds__(COMPACT_T) k_compact_cells(const SceneDesc* __restrict__ Dsc, int which) {
   AGIMP_SCENE
-  const uint8_t* flags = T.cflag;
+  __shared__ int wsum[COMPACT_T / 64];
+  __shared__ int wgbase;
   const int n = T.total_blocks;
+  const int nw = (n + 3) >> 2;
   int* list = T.clist;
-  int* count = T.ccount + which;
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  bool on = g < n && flags[g] != 0;
-  int idx = wave_agg_atomic_add(count, on);
-  if (on) list[idx] = g;
+  const int w = blockIdx.x * COMPACT_T + threadIdx.x;
+  const uint32_t word = w < nw ? ((const uint32_t AGIMP_GLOBAL*)T.cflag)[w] : 0u;
+  const int g0 = w * 4;
+  int c = 0;
+#pragma unroll
+  for (int b = 0; b < 4; ++b) c += (((word >> (8 * b)) & 0xffu) != 0u) && g0 + b < n;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  int incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+#pragma unroll
+    for (int i = 0; i < COMPACT_T / 64; ++i) {
+      const int v = wsum[i];
+      wsum[i] = tot;
+      tot += v;
+    }
+    wgbase = tot ? atomicAdd(T.ccount + which, tot) : 0;
+  }
+  __syncthreads();
+  int idx = wgbase + wsum[wave] + incl - c;
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+    if ((((word >> (8 * b)) & 0xffu) != 0u) && g0 + b < n) list[idx++] = g0 + b;
 }
 
 // ------------------------------------------------------------------------------ clear
@@ -874,20 +904,38 @@ __global__ void k_scatter_blocks(const SceneDesc* __restrict__ Dsc) {
 
 // compact the grid-phase block flags of every level into per-level lists
 template <int D>
-__global__ void k_scompact(const SceneDesc* __restrict__ Dsc) {
+__global__ void __launch_bounds__(COMPACT_T) k_scompact(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
-  if (blockIdx.x * blockDim.x >= (unsigned)T.total_blocks) return;  // whole workgroup past this scene
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = g < T.total_blocks;
-  int l = 0, b = 0;
-  bool on = false;
-  if (live) {
-    l = level_of_gbid(T, g);
-    b = g - T.lv[l].gbase;
-    on = T.lv[l].sflag[b] != 0;
+  // each thread reads one word of 4 block flags (global block ids 4w .. 4w+3, levels laid out
+  // one after the other); slots are counted per level in LDS and reserved with one atomic per
+  // (workgroup, level)
+  __shared__ int lcnt[MAXL], lbase[MAXL];
+  const int n = T.total_blocks;
+  const int nw = (n + 3) >> 2;
+  if (blockIdx.x * COMPACT_T >= (unsigned)nw) return;  // whole workgroup past this scene
+  if (threadIdx.x < MAXL) lcnt[threadIdx.x] = 0;
+  __syncthreads();
+  const int w = blockIdx.x * COMPACT_T + threadIdx.x;
+  const uint32_t word = w < nw ? ((const uint32_t AGIMP_GLOBAL*)T.lv[0].sflag)[w] : 0u;
+  int lv[4], off[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    lv[k] = -1;
+    const int g = 4 * w + k;
+    if (((word >> (8 * k)) & 0xffu) != 0u && g < n) {
+      lv[k] = level_of_gbid(T, g);
+      off[k] = atomicAdd(&lcnt[lv[k]], 1);
+    }
   }
-  int idx = wave_agg_atomic_add(&T.scount[l], on);
-  if (on) T.lv[l].slist[idx] = b;
+  __syncthreads();
+  if (threadIdx.x < MAXL) {
+    const int c = lcnt[threadIdx.x];
+    lbase[threadIdx.x] = c ? atomicAdd(&T.scount[threadIdx.x], c) : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (lv[k] >= 0) T.lv[lv[k]].slist[lbase[lv[k]] + off[k]] = 4 * w + k - T.lv[lv[k]].gbase;
 }
 
 // ------------------------------------------------------------------------------ wave reductions
@@ -1694,7 +1742,7 @@ static inline dim3 grid2(size_t x, int nscenes) {
 
 void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
-  const dim3 bg = grid2(blocks_for(m.max_total_blocks, 256), m.nscenes);
+  const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
   const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
   AGIMP_DIM(m.dim, {
@@ -1705,13 +1753,13 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
       hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
       if (learn) launch_unit_activate(s, d, m.dim, (int)m.max_n, par, *learn);
       if (m.maxL > 1) {
-        hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 0);
+        hipLaunchKernelGGL(k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 0);
         hipLaunchKernelGGL(k_mark_hier<DD>, lg, dim3(256), 0, s, d);
-        hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 1);
+        hipLaunchKernelGGL(k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 1);
         hipLaunchKernelGGL(k_split<DD>, lg, dim3(256), 0, s, d);
       }
     }
-    hipLaunchKernelGGL(k_compact_cells, bg, dim3(256), 0, s, d, 2);
+    hipLaunchKernelGGL(k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 2);
     hipLaunchKernelGGL(k_node_A<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_mark_T<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_mark_G<DD>, lg, dim3(256), 0, s, d);
@@ -1723,7 +1771,7 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
 
 void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
-  const dim3 bg = grid2(blocks_for(m.max_total_blocks, 256), m.nscenes);
+  const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
   const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
@@ -1734,7 +1782,7 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
     hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_scompact<DD>, bg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_scompact<DD>, cg, dim3(COMPACT_T), 0, s, d);
   });
 }
 
