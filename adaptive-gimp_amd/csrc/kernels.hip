@@ -782,20 +782,28 @@ constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 
 __device__ __forceinline__ int2 add2(int2 a, int2 b) { return make_int2(a.x + b.x, a.y + b.y); }
 
+// exclusive scan over the workgroup: shuffles inside each wavefront, then the SCAN_T / 64
+// wavefront totals through LDS (sh holds at least SCAN_T / 64 entries)
 __device__ __forceinline__ int2 block_excl_scan(int2 v, int2* sh, int2& total) {
-  int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int off = 1; off < SCAN_T; off <<= 1) {
-    int2 o = (tid >= off) ? sh[tid - off] : make_int2(0, 0);
-    __syncthreads();
-    sh[tid] = add2(sh[tid], o);
-    __syncthreads();
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  int2 incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int ux = __shfl_up(incl.x, o), uy = __shfl_up(incl.y, o);
+    if (lane >= o) { incl.x += ux; incl.y += uy; }
   }
-  int2 incl = sh[tid];
-  total = sh[SCAN_T - 1];
+  __syncthreads();  // sh may still be read by a previous call
+  if (lane == 63) sh[wave] = incl;
   __syncthreads();
-  return make_int2(incl.x - v.x, incl.y - v.y);
+  int2 off = make_int2(0, 0);
+  total = make_int2(0, 0);
+#pragma unroll
+  for (int i = 0; i < SCAN_T / 64; ++i) {
+    const int2 t = sh[i];
+    if (i < wave) off = add2(off, t);
+    total = add2(total, t);
+  }
+  return make_int2(off.x + incl.x - v.x, off.y + incl.y - v.y);
 }
 
 __device__ __forceinline__ int2 load_cc(const int* bcnt, int i, int n, int CH) {
