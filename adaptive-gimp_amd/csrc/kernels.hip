@@ -136,15 +136,37 @@ __device__ __forceinline__ void fill_zero(uint32_t* p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += st) p[i] = 0u;
 }
 
-// before refine: cell masks, node mask, block flags, list counters
+// Before refine: cell masks, node mask, block flags, list counters.  Only flagged blocks can
+// hold marks: every cell write flags its leaf block (cflag), and every node mark lies in a
+// finest-level block whose grid-phase flag (sflag) is set (the node marks set it, it is saved
+// after refine and restored, never cleared, by bucketing).  One thread per global block
+// zeroes the block's cells if cflag is set and, for finest blocks with sflag set, its nodes;
+// then both flags.  A thread reads and clears only its own block's flags.
+template <int D>
 __global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
-  fill_zero((uint32_t*)T.lv[0].cell, sd.w_cells);
-  fill_zero((uint32_t*)T.node, sd.w_node);
-  fill_zero((uint32_t*)T.lv[0].sflag, sd.w_flags);
-  fill_zero((uint32_t*)T.cflag, sd.w_flags);
-  fill_zero((uint32_t*)T.scount, MAXL);
-  fill_zero((uint32_t*)T.ccount, 4);
+  constexpr int WPB = Pow4<D>::v / 4;  // 32-bit words per leaf block of byte masks
+  const int nb = T.total_blocks, fbase = T.lv[T.L - 1].gbase;
+  uint8_t AGIMP_GLOBAL* sflag = T.lv[0].sflag;
+  uint32_t AGIMP_GLOBAL* cells = (uint32_t AGIMP_GLOBAL*)T.lv[0].cell;  // block g at word g * WPB
+  uint32_t AGIMP_GLOBAL* nodes = (uint32_t AGIMP_GLOBAL*)T.node;
+  const int st = gridDim.x * blockDim.x;
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nb; g += st) {
+    if (T.cflag[g]) {
+#pragma unroll
+      for (int w = 0; w < WPB; ++w) cells[(size_t)g * WPB + w] = 0u;
+      T.cflag[g] = 0;
+    }
+    if (sflag[g]) {
+      if (g >= fbase) {
+#pragma unroll
+        for (int w = 0; w < WPB; ++w) nodes[(size_t)(g - fbase) * WPB + w] = 0u;
+      }
+      sflag[g] = 0;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < 4) T.ccount[threadIdx.x] = 0;
 }
 
 // the valid-node block flags of the octree: save after refine (to_saved) or restore
@@ -534,6 +556,7 @@ __device__ __forceinline__ void mark_G_face(const Topo& T, unsigned long long* c
     int64_t qc = lidx<D>(T.lv[l0], P);
     if (qc < 0) { count(cnt, CNT_OOB); continue; }
     T.lv[l0].cell[qc] |= GHOST;
+    mark_cell_block<D>(T, l0, qc);
 #pragma unroll
     for (int u = 0; u < (1 << D); ++u) {
       int f[3];
@@ -1751,11 +1774,12 @@ static inline dim3 grid2(size_t x, int nscenes) {
 void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
+  const dim3 rg = grid2(std::min<size_t>(blocks_for(m.max_total_blocks, 256), 4096), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
   const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
   AGIMP_DIM(m.dim, {
     hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_clear_refine, fg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_clear_refine<DD>, rg, dim3(256), 0, s, d);
     if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
     if (m.any_dynamic) {
       hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
