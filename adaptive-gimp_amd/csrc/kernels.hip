@@ -286,14 +286,33 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
   const PBuf& P = sd.P[par];
   const int n = sd.n;
   unsigned long long* cnt = sd.cnt;
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int g = P.g[k];
-  if (g < 0 || g >= T.L) return;
-  const LevelDesc Lg = T.lv[g];
-  int b[3];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = k < n ? P.g[k] : -1;
+  const bool live = g >= 0 && g < T.L;
+  const LevelDesc Lg = T.lv[live ? g : 0];
+  int b[3] = {0, 0, 0};
+  if (live)
 #pragma unroll
-  for (int a = 0; a < D; ++a) b[a] = base_cell(P.f[(size_t)a * P.stride + k], Lg, T.pow2);
+    for (int a = 0; a < D; ++a) b[a] = base_cell(P.f[(size_t)a * P.stride + k], Lg, T.pow2);
+  // particles arrive sorted by cell: a lane whose left neighbour requests the same cells
+  // (same level and base cell) leaves the stores to it; out-of-range cells are still counted
+  // per particle.  Every lane reaches the shuffle.
+  bool packs = live;  // base cells in [-2, 2^20 - 2): the 20-bit fields of the key are exact
+#pragma unroll
+  for (int a = 0; a < D; ++a) packs = packs && b[a] >= -2 && b[a] < (1 << 20) - 2;
+  const unsigned long long key =
+      packs ? (((unsigned long long)g << 60) | ((unsigned long long)(b[0] + 2) << 40) |
+               ((unsigned long long)(b[1] + 2) << 20) | (unsigned long long)(D == 3 ? b[2] + 2 : 0))
+            : ~(unsigned long long)__lane_id();
+  const unsigned long long left = __shfl_up(key, 1);
+  const bool dup = __lane_id() > 0 && left == key;
+  if (!live) return;
+  // corners whose block equals that of the corner with a lower bit on an axis where b and b+1
+  // share a block flag nothing new
+  int cross = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    if (((b[a] + PAD) >> 2) != ((b[a] + 1 + PAD) >> 2)) cross |= 1 << (D - 1 - a);
 #pragma unroll
   for (int t = 0; t < (1 << D); ++t) {
     int I[3];
@@ -301,8 +320,9 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
     for (int a = 0; a < D; ++a) I[a] = b[a] + ((t >> (D - 1 - a)) & 1);
     int64_t q = lidx<D>(Lg, I);
     if (q < 0) { count(cnt, CNT_OOB); continue; }
+    if (dup) continue;
     Lg.cell[q] = ACTIVATED;  // cells only ever receive ACTIVATED in this phase
-    mark_cell_block<D>(T, g, q);
+    if ((t & ~cross) == 0) mark_cell_block<D>(T, g, q);
   }
 }
 
