@@ -96,6 +96,23 @@ __device__ __forceinline__ USeg useg(int key) {
 __device__ __forceinline__ float op_add(float a, float b) { return a + b; }
 __device__ __forceinline__ float op_max(float a, float b) { return fmaxf(a, b); }
 
+// a lane's request key: level g and the base cell of x at that level (20 bits per axis),
+// or a value unique to the lane when the base cell does not fit the fields
+template <int D>
+__device__ __forceinline__ unsigned long long request_key(int g, const float* x, const LevelDesc& Lg, int pow2,
+                                                          int lane) {
+  int b[3] = {0, 0, 0};
+  bool packs = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    b[a] = base_cell(x[a], Lg, pow2);
+    packs = packs && b[a] >= -2 && b[a] < (1 << 20) - 2;
+  }
+  if (!packs) return ~(unsigned long long)lane;
+  return ((unsigned long long)g << 60) | ((unsigned long long)(b[0] + 2) << 40) |
+         ((unsigned long long)(b[1] + 2) << 20) | (unsigned long long)(D == 3 ? b[2] + 2 : 0);
+}
+
 // the particle's requests (adaptive_gimp.py:87-96): its 2^d cells at level g; record g+1 in
 // the units they lie in, for g >= ulev (coarser requests are not predicted)
 template <int D>
@@ -196,8 +213,11 @@ __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, Le
   int u = -1, g = -1;
   float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
   float m = 0.f, vn = 0.f, cn = 0.f, jm1 = 0.f, njac = 0.f;
+  float xf[3] = {0.f, 0.f, 0.f};
+  int Iown[3] = {0, 0, 0};
+  unsigned long long k1 = ~(unsigned long long)__lane_id(), k2 = k1;  // unique unless set below
   if (live) {
-    int I[3], Iown[3];
+    int I[3];
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       x[a] = P.f[(size_t)(PL::X + a) * P.stride + k];
@@ -229,15 +249,27 @@ __global__ void k_unit_accumulate(const SceneDesc* __restrict__ Dsc, int par, Le
     cn = sqrtf(C2);
     jm1 = J - 1.f;
     njac = -J;
-    // request cells lie within one unit of the particle's unit: all are candidates
-    if (u >= 0) {
-      record_requests<D>(T, U.ulev, x, g, reinterpret_cast<int*>(U.acc) + UA_REQ, UNIT_ACC);
-      // the same requests after a straight move over the window (one rounding: fma)
-      float xf[3];
+    // the same requests after a straight move over the window (one rounding: fma)
 #pragma unroll
-      for (int a = 0; a < D; ++a) xf[a] = __fmaf_rn(v[a], U.win, x[a]);
-      record_requests<D>(T, U.ulev, xf, g, reinterpret_cast<int*>(U.acc) + UA_REQF, UNIT_ACC, Iown);
+    for (int a = 0; a < D; ++a) xf[a] = __fmaf_rn(v[a], U.win, x[a]);
+    if (u >= 0 && g >= U.ulev && g < T.L) {
+      const LevelDesc Lg = T.lv[g];
+      k1 = request_key<D>(g, x, Lg, T.pow2, __lane_id());
+      k2 = request_key<D>(g, xf, Lg, T.pow2, __lane_id());
     }
+  }
+  // Particles arrive sorted by cell, so neighbours mostly record the same requests: a lane
+  // whose left neighbour has the same key (level and base cell; for the swept requests also
+  // the same unit, which bounds the recorded units) leaves the records to it.  Every lane
+  // reaches the shuffles.
+  const unsigned long long l1 = __shfl_up(k1, 1), l2 = __shfl_up(k2, 1);
+  const int lu = __shfl_up(u, 1);
+  const bool first = __lane_id() > 0;
+  if (live && u >= 0) {
+    // request cells lie within one unit of the particle's unit: all are candidates
+    if (!(first && l1 == k1)) record_requests<D>(T, U.ulev, x, g, reinterpret_cast<int*>(U.acc) + UA_REQ, UNIT_ACC);
+    if (!(first && l2 == k2 && lu == u))
+      record_requests<D>(T, U.ulev, xf, g, reinterpret_cast<int*>(U.acc) + UA_REQF, UNIT_ACC, Iown);
   }
   const bool in = live && u >= 0;
   const int key = in ? u : -2 - (int)__lane_id();
