@@ -103,14 +103,14 @@ class AdaptiveGIMP:
     def __init__(self, dim, level, coarsest_size, n_particles, particle_initializer, grid_initializer=None, *,
                  E=5e3, nu=0.2, gravity=None, p_mass=1.0, p_vol=1.0, boundary_gap=4, wall_mode="slip",
                  plastic=False, theta_c=2.5e-2, theta_s=7.5e-3, hardening=0.0, slab=None,
-                 particles_per_chunk=1024, device=0):
+                 particles_per_chunk=None, device=0):
         if dim not in (2, 3):
             raise ValueError("dim must be 2 or 3")
         self._kwargs = dict(E=E, nu=nu, gravity=None if gravity is None else [float(v) for v in gravity],
                             p_mass=p_mass, p_vol=p_vol, boundary_gap=boundary_gap, wall_mode=wall_mode,
                             plastic=bool(plastic), theta_c=theta_c, theta_s=theta_s, hardening=hardening,
                             slab=None if slab is None else [int(slab[0]), float(slab[1]), float(slab[2])],
-                            particles_per_chunk=particles_per_chunk)
+                            particles_per_chunk=None if particles_per_chunk is None else int(particles_per_chunk))
         self.dim = dim
         self.level = level
         self.coarsest_size = coarsest_size
@@ -143,7 +143,8 @@ class AdaptiveGIMP:
         cfg.theta_c, cfg.theta_s, cfg.hardening = theta_c, theta_s, hardening
         if slab is not None:
             cfg.slab_enable, cfg.slab_axis, cfg.slab_lo, cfg.slab_hi = 1, int(slab[0]), float(slab[1]), float(slab[2])
-        cfg.particles_per_chunk = int(particles_per_chunk)
+        if particles_per_chunk is not None:  # else the library default (2048 in 3D, 1024 in 2D)
+            cfg.particles_per_chunk = int(particles_per_chunk)
         cfg.device = int(device)
         self._cfg = cfg
         h = ctypes.c_void_p()

@@ -105,7 +105,7 @@ def _threads():
 
 def _new_sim(pkg, wl, ppc):
     return pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(), wl.grid_initializer,
-                            particles_per_chunk=ppc, **wl.params)
+                            particles_per_chunk=ppc or None, **wl.params)
 
 
 def _learned_sim(pkg, wl, ppc, model, parts, tau=0.3, window=0):
@@ -176,7 +176,8 @@ def accuracy(pkg, wl, args, np, model):
 
 def bench_single(pkg, wl, args, np):
     sim = pkg.AdaptiveGIMP(wl.dim, wl.level, wl.coarsest_size, wl.n, wl.initializer(), wl.grid_initializer,
-                           particles_per_chunk=args.ppc, **wl.params)
+                           particles_per_chunk=args.ppc or None, **wl.params)
+    args.ppc_used = int(sim._cfg.particles_per_chunk)
     tuned = None
     if args.tune:
         from importlib import import_module
@@ -241,7 +242,9 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="particle count override (c3/c5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the P2G chunk size first")
-    ap.add_argument("--ppc", type=int, default=1024, help="P2G particles per chunk (single-scene workloads)")
+    ap.add_argument("--ppc", type=int, default=0,
+                    help="P2G particles per chunk (single-scene workloads; 0: the library default, 2048 in 3D, "
+                         "1024 in 2D)")
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"],
                     help="single-scene workloads: time the plain path, the learned path (predicted octrees, "
                          "adaptive-gimp_amd/refine_mlp.npz), or both (value = the plain path)")
@@ -312,7 +315,7 @@ def main():
             "ms_per_step": elapsed * 1e3 / args.steps,
             "config": {"workload": wl.name, "dim": wl.dim, "levels": wl.level, "coarsest_size": wl.coarsest_size,
                        "n_particles": wl.n, "dt": wl.dt, "mode": "plain", "timed_path": "hipGraph replay",
-                       "particles_per_chunk": tuned[0] if tuned else args.ppc},
+                       "particles_per_chunk": tuned[0] if tuned else args.ppc_used},
             "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": measured_traffic(dom) if wl.name.startswith("c3") else None,
