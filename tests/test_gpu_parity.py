@@ -378,3 +378,47 @@ def test_render_matches_oracle_octree(pkg, O, name, tmp_path):
     with_p = R.render(sim, res=res)
     assert (with_p != got).any(axis=2).sum() > 0
     sim.close()
+
+
+def test_static_many_substeps_track_oracle(pkg, O):
+    """Static adaptivity (run.py --case 3) over 6 substeps: every substep rebuilds the masks
+    from the grid initializer, so the topology stays bit-exact and particles track the oracle
+    within the multi-substep bars."""
+    st = ref_state(pkg.scenes.reference_case0_2d(3), 20000, 2)
+    gi = pkg.scenes.REFERENCE_GRID_INITIALIZERS[3]
+    sim, ora = make_pair(pkg, O, 2, 4, 16, st, {}, gi)
+    for _ in range(6):
+        sim.substep(2e-5)
+        ora.substep(2e-5)
+    assert_topology_equal(sim, ora, O, 4)
+    e = particle_errors(sim, ora)
+    assert e["fl"] == 0 and e["x"] <= 1e-5 and e["v"] <= 1e-3 and e["F"] <= 1e-4, e
+    sim.close()
+
+
+def test_batch_matches_single_scenes(pkg, O):
+    """Batch: three 2D scenes of different sizes (two dynamic, one static) advanced together
+    for 6 substeps equal the same scenes advanced one by one: octrees bit for bit, particles to
+    FP32 rounding (float atomics sum in another order)."""
+    S = pkg.scenes
+    specs = [(S.reference_case0_2d(0, seed=1), 6000, None), (S.reference_case0_2d(0, seed=2), 15000, None),
+             (S.reference_case0_2d(2, seed=3), 9000, S.REFERENCE_GRID_INITIALIZERS[2])]
+    mk = lambda init, n, gi: pkg.AdaptiveGIMP(2, 4, 16, n, init, gi)
+    batch_sims = [mk(*sp) for sp in specs]
+    single = [mk(*sp) for sp in specs]
+    b = pkg.Batch(batch_sims)
+    b.advance(2e-5, 6)
+    for s in single:
+        s.advance(2e-5, 6)
+    for a, s in zip(batch_sims, single):
+        for l in range(4):
+            assert np.array_equal(a.cell_mask[l].to_numpy(), s.cell_mask[l].to_numpy()), l
+        assert np.array_equal(O.canonical_node_mask(a.node_mask.to_numpy()), O.canonical_node_mask(s.node_mask.to_numpy()))
+        xa, xs = a.x_p.to_numpy(), s.x_p.to_numpy()
+        va, vs = a.v_p.to_numpy(), s.v_p.to_numpy()
+        assert np.abs(xa - xs).max() <= 1e-6
+        assert np.abs(va - vs).max() <= 1e-4 * max(np.abs(vs).max(), 1e-30)
+        assert np.array_equal(a.fl_p.to_numpy(), s.fl_p.to_numpy())
+    b.close()
+    for s in batch_sims + single:
+        s.close()
