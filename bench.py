@@ -236,9 +236,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c3", choices=["c3", "c5", "c2", "c4"],
+    ap.add_argument("--workload", default="c3", choices=["c3", "c5", "c2", "c4", "c1"],
                     help="c3: 3D 1M sphere (the headline); c5: 3D 4M three spheres; c2: 2D static band; "
-                         "c4: 64 batched 2D scenes")
+                         "c4: 64 batched 2D scenes; c1: 2D two squares (~21k particles)")
     ap.add_argument("--n", type=int, default=0, help="particle count override (c3/c5)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the P2G chunk size first")
@@ -289,6 +289,9 @@ def main():
     elif args.workload == "c2":
         wls = [S.c2_static_band(rho=1000.0)]
         notes = "rho=1000 instead of the stated 1 (rho=1 violates the P-wave CFL bound at dt=5e-5)"
+    elif args.workload == "c1":
+        wls = [S.c1_two_squares(rho=1000.0)]
+        notes = "rho=1000 instead of the stated 1 (rho=1 violates the P-wave CFL bound at dt=1e-4)"
     else:
         wls = S.c4_batch(rho=1000.0)
         notes = "rho=1000 instead of the stated 1 (rho=1 violates the P-wave CFL bound at dt=5e-5)"
