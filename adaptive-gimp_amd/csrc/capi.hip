@@ -200,6 +200,7 @@ Dims single_dims(const agimp_handle* h) {
   m.max_total_blocks = (size_t)h->hd.T.total_blocks;
   m.max_fill_words = h->hd.w_cells + h->hd.w_node + 2 * h->hd.w_flags;
   m.list_grid = 2048;
+  m.fill_grid = 4096;
   m.p2g_grid = std::max(1, std::min(h->max_chunks, 8192));
   m.grid_grid = 4096;
   return m;
@@ -970,6 +971,7 @@ int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out
   m.dim = scenes[0] ? scenes[0]->D : 0;
   m.nscenes = n;
   m.list_grid = std::max(16, 2048 / n);
+  m.fill_grid = std::max(16, 4096 / n);  // the grid-stride loops absorb the rest
   m.grid_grid = std::max(16, 4096 / n);
   int p2g = 0;
   for (int i = 0; i < n; ++i) {

@@ -1794,9 +1794,9 @@ static inline dim3 grid2(size_t x, int nscenes) {
 void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
-  const dim3 rg = grid2(std::min<size_t>(blocks_for(m.max_total_blocks, 256), 4096), m.nscenes);
+  const dim3 rg = grid2(std::min<size_t>(blocks_for(m.max_total_blocks, 256), (size_t)m.fill_grid), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
-  const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
+  const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), (size_t)m.fill_grid), m.nscenes);
   AGIMP_DIM(m.dim, {
     hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_clear_refine<DD>, rg, dim3(256), 0, s, d);
@@ -1825,7 +1825,7 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
-  const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), 4096), m.nscenes);
+  const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), (size_t)m.fill_grid), m.nscenes);
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
   const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {

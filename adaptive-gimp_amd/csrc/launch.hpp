@@ -17,6 +17,7 @@ struct Dims {
   size_t max_total_blocks;  // largest number of leaf blocks over all levels
   size_t max_fill_words;    // largest per-substep clear (32-bit words)
   int list_grid;            // workgroups per scene for block-list kernels
+  int fill_grid;            // largest grid per scene of the clear / flag-copy kernels
   int p2g_grid;             // workgroups per scene for the P2G chunk loop
   int grid_grid;            // workgroups per scene for the grid passes
 };
