@@ -110,7 +110,8 @@ class AdaptiveGIMP:
                             p_mass=p_mass, p_vol=p_vol, boundary_gap=boundary_gap, wall_mode=wall_mode,
                             plastic=bool(plastic), theta_c=theta_c, theta_s=theta_s, hardening=hardening,
                             slab=None if slab is None else [int(slab[0]), float(slab[1]), float(slab[2])],
-                            particles_per_chunk=None if particles_per_chunk is None else int(particles_per_chunk))
+                            particles_per_chunk=None if particles_per_chunk is None else int(particles_per_chunk),
+                            device=int(device))
         self.dim = dim
         self.level = level
         self.coarsest_size = coarsest_size
