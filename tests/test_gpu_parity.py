@@ -422,3 +422,29 @@ def test_batch_matches_single_scenes(pkg, O):
     b.close()
     for s in batch_sims + single:
         s.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_refine_after_moving_everything_leaves_no_stale_marks(pkg, O, dim):
+    """The rebuild clears only blocks flagged by the previous octree: after all particles move
+    to a disjoint region, the new octree equals the oracle's for the new positions alone."""
+    rng = np.random.default_rng(5 + dim)
+    n, L, c = 4000, 3, 16
+    def state(lo):
+        x = lo + 0.25 * rng.random((n, dim))
+        g = np.where(rng.random(n) < 0.5, 2, np.where(rng.random(n) < 0.5, 1, 0)).astype(np.int32)
+        return dict(x=x, v=np.zeros((n, dim)), F=np.tile(np.eye(dim), (n, 1, 1)), C=np.zeros((n, dim, dim)),
+                    m=np.ones(n), g=g)
+    a, b = state(0.1), state(0.6)
+    sim, _ = make_pair(pkg, O, dim, L, c, a, {})
+    sim.refine()
+    assert sim.cell_mask[L - 1].to_numpy().any()
+    sim.x_p.from_numpy(b["x"])
+    sim.g_p.from_numpy(b["g"])
+    sim.refine()
+    ora = O.OracleSim(dim, L, c, n, fp64=False)
+    f32 = lambda v: np.asarray(v, np.float32)
+    ora.set_particles(f32(b["x"]), f32(b["v"]), f32(b["F"]), f32(b["C"]), f32(b["m"]), b["g"])
+    ora.refine()
+    assert_topology_equal(sim, ora, O, L)
+    sim.close()
