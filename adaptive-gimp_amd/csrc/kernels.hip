@@ -1001,21 +1001,32 @@ __device__ __forceinline__ float dpp_f(float src) {
 }
 
 // Which DPP steps of a segmented inclusive scan add their source: lanes sorted by key form
-// contiguous segments; a step adds only when the source lane carries the same key.  The scan
-// stays inside 16-lane rows: the last lane of each row piece of a segment ("tail") holds that
-// piece's sum (a segment spanning rows has one tail per row).
+// contiguous segments; a step adds only when the source lane carries the same key.  With
+// WAVE, the scan spans the wavefront (row_shr inside 16-lane rows, then the row_bcast steps
+// across rows) and each segment has one tail, its last lane, holding the segment's sum.
+// Without, it stays inside rows: the last lane of each row piece of a segment is a tail
+// holding that piece's sum (a segment spanning rows has one tail per row).
 struct SegMask {
-  bool s1, s2, s4, s8, tail;
+  bool s1, s2, s4, s8, s16, s32, tail;
 };
 
+template <bool WAVE>
 __device__ __forceinline__ SegMask seg_mask(int key) {
   SegMask m;
   m.s1 = dpp_i<0x111, 0xf>(key, -1) == key;
   m.s2 = dpp_i<0x112, 0xf>(key, -1) == key;
   m.s4 = dpp_i<0x114, 0xf>(key, -1) == key;
   m.s8 = dpp_i<0x118, 0xf>(key, -1) == key;
-  int nxt = __shfl_down(key, 1);
-  m.tail = ((__lane_id() & 15) == 15) || (nxt != key);
+  m.s16 = m.s32 = false;
+  const int nxt = __shfl_down(key, 1);
+  if constexpr (WAVE) {
+    // lane 15 of the row below (rows 1 and 3 take it) and lane 31 (rows 2 and 3)
+    m.s16 = dpp_i<0x142, 0xa>(key, -1) == key;
+    m.s32 = dpp_i<0x143, 0xc>(key, -1) == key;
+    m.tail = (__lane_id() == 63) || (nxt != key);
+  } else {
+    m.tail = ((__lane_id() & 15) == 15) || (nxt != key);
+  }
   return m;
 }
 
@@ -1028,12 +1039,24 @@ __device__ __forceinline__ float seg_step(float v, bool on) {
   return on ? s : v;
 }
 
-// Segmented inclusive scan within rows: each tail ends up holding its row piece's sum.
+// one step across rows (row_bcast): the broadcast lane's value, where the rows take it
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ float seg_step_rows(float v, bool on) {
+  const float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, false));
+  return on ? v + t : v;
+}
+
+// Segmented inclusive scan: each tail ends up holding its segment's (or row piece's) sum.
+template <bool WAVE>
 __device__ __forceinline__ float seg_sum(float v, const SegMask& m) {
   v = seg_step<0x111>(v, m.s1);
   v = seg_step<0x112>(v, m.s2);
   v = seg_step<0x114>(v, m.s4);
   v = seg_step<0x118>(v, m.s8);
+  if constexpr (WAVE) {
+    v = seg_step_rows<0x142, 0xa>(v, m.s16);
+    v = seg_step_rows<0x143, 0xc>(v, m.s32);
+  }
   return v;
 }
 
@@ -1057,10 +1080,12 @@ constexpr int P2G_MAX_CH = 2048; // largest work chunk (particles of one leaf bl
 //  1. the chunk is counting-sorted by cell inside the block (LDS histogram); the order is
 //     written back to perm, so G2P stores particles cell-sorted for the next substep;
 //  2. each lane computes its particle's 3^d node contributions; lanes of one cell are summed
-//     with a segmented DPP scan inside each 16-lane row, and only the last lane of each row
-//     piece adds into the (4+2)^d LDS node tile (at most 4 lanes of one instruction hit the
-//     same LDS word);
-//  3. the tile is flushed with one float atomic per non-zero node component.
+//     with a segmented DPP scan and only the scan's tails add into the (4+2)^d LDS node tile.
+//     3D: the scan spans the wavefront, so one lane per cell and wavefront writes, and each
+//     wavefront owns a tile that it updates by plain read-add-write.  2D: the scan stays in
+//     16-lane rows and the row tails add with LDS float atomics into one shared tile (at most
+//     4 lanes of one instruction hit the same LDS word);
+//  3. the tiles are summed and flushed with one float atomic per non-zero node component.
 template <int D>
 __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
   AGIMP_SCENE
@@ -1075,7 +1100,14 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
   constexpr int TN = (D == 2) ? TS * TS : TS * TS * TS;
   constexpr int NC = Pow4<D>::v;  // cells per leaf block
   constexpr int PER = P2G_MAX_CH / P2G_T;
-  __shared__ float tile[TN * 4];
+  // 3D: one node tile per wavefront and one tail per cell and wavefront, so a tail adds its
+  // sums with a plain read-add-write of the node's float4 (LDS operations of a wavefront run
+  // in order; tails of one instruction belong to different cells, hence different nodes).
+  // 2D: one tile, row-local tails, LDS float atomics (the extra scan steps cost more there
+  // than the 3 atomics per node they would save).
+  constexpr bool WAVE = (D == 3);
+  constexpr int NWV = WAVE ? P2G_T / 64 : 1;
+  __shared__ float4 tile[NWV][TN];
   __shared__ int hist[NC];
   __shared__ int sidx[P2G_MAX_CH];
   const int nchunks = totals[1];
@@ -1086,7 +1118,7 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
     const LevelDesc Ll = T.lv[l];
     int B[3];
     block_coords<D>(Ll.NB, ck.x - Ll.gbase, B);
-    for (int i = tid; i < TN * 4; i += P2G_T) tile[i] = 0.f;
+    for (int i = tid; i < NWV * TN; i += P2G_T) (&tile[0][0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tid < NC) hist[tid] = 0;
     __syncthreads();
     // 1. counting sort of the chunk by cell
@@ -1166,7 +1198,7 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
         bspline(tt[a], w[a], dw[a]);
       }
       if (!valid) key = 4096 + __lane_id();  // unique: never joins a neighbour's segment
-      const SegMask sm = seg_mask(key);
+      const SegMask sm = seg_mask<WAVE>(key);
       const bool writer = valid && sm.tail;
       float S[3][3];
 #pragma unroll
@@ -1191,14 +1223,14 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
             float mw = wt * mp;
             float p0 = mw * (vb[0] + Cm[0][0] * i0 + Cm[0][1] * i1) + S[0][0] * g0 + S[0][1] * g1;
             float p1 = mw * (vb[1] + Cm[1][0] * i0 + Cm[1][1] * i1) + S[1][0] * g0 + S[1][1] * g1;
-            p0 = seg_sum(p0, sm);
-            p1 = seg_sum(p1, sm);
-            mw = seg_sum(mw, sm);
+            p0 = seg_sum<WAVE>(p0, sm);
+            p1 = seg_sum<WAVE>(p1, sm);
+            mw = seg_sum<WAVE>(mw, sm);
             if (writer) {
-              int nd = ((lb[0] + i0) * TS + (lb[1] + i1)) * 4;
-              atomicAdd(&tile[nd + 0], p0);
-              atomicAdd(&tile[nd + 1], p1);
-              atomicAdd(&tile[nd + 3], mw);
+              float4& t = tile[0][(lb[0] + i0) * TS + (lb[1] + i1)];
+              atomicAdd(&t.x, p0);
+              atomicAdd(&t.y, p1);
+              atomicAdd(&t.w, mw);
             }
           }
       } else {
@@ -1236,16 +1268,15 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
                 if (i2 == 2) pa += mw * C2[a];
                 p[a] = pa;
               }
-              p[0] = seg_sum(p[0], sm);
-              p[1] = seg_sum(p[1], sm);
-              p[2] = seg_sum(p[2], sm);
-              mw = seg_sum(mw, sm);
+              p[0] = seg_sum<WAVE>(p[0], sm);
+              p[1] = seg_sum<WAVE>(p[1], sm);
+              p[2] = seg_sum<WAVE>(p[2], sm);
+              mw = seg_sum<WAVE>(mw, sm);
               if (writer) {
-                int nd = (((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)) * 4;
-                atomicAdd(&tile[nd + 0], p[0]);
-                atomicAdd(&tile[nd + 1], p[1]);
-                atomicAdd(&tile[nd + 2], p[2]);
-                atomicAdd(&tile[nd + 3], mw);
+                float4& t = tile[tid >> 6][((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)];
+                float4 u = t;
+                u.x += p[0]; u.y += p[1]; u.z += p[2]; u.w += mw;
+                t = u;
               }
             }
           }
@@ -1254,7 +1285,12 @@ __global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc
     __syncthreads();
     // 3. flush
     for (int i = tid; i < TN; i += P2G_T) {
-      float4 val = make_float4(tile[i * 4], tile[i * 4 + 1], tile[i * 4 + 2], tile[i * 4 + 3]);
+      float4 val = tile[0][i];
+#pragma unroll
+      for (int w = 1; w < NWV; ++w) {
+        const float4 o = tile[w][i];
+        val.x += o.x; val.y += o.y; val.z += o.z; val.w += o.w;
+      }
       if (val.x == 0.f && val.y == 0.f && val.z == 0.f && val.w == 0.f) continue;
       int J[3];
       if (D == 2) { J[0] = 4 * B[0] + i / TS; J[1] = 4 * B[1] + i % TS; }
