@@ -1039,13 +1039,6 @@ __device__ __forceinline__ float seg_step(float v, bool on) {
   return on ? s : v;
 }
 
-// one step across rows (row_bcast): the broadcast lane's value, where the rows take it
-template <int CTRL, int ROWMASK>
-__device__ __forceinline__ float seg_step_rows(float v, bool on) {
-  const float t = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWMASK, 0xf, false));
-  return on ? v + t : v;
-}
-
 // Segmented inclusive scan: each tail ends up holding its segment's (or row piece's) sum.
 template <bool WAVE>
 __device__ __forceinline__ float seg_sum(float v, const SegMask& m) {
@@ -1054,8 +1047,11 @@ __device__ __forceinline__ float seg_sum(float v, const SegMask& m) {
   v = seg_step<0x114>(v, m.s4);
   v = seg_step<0x118>(v, m.s8);
   if constexpr (WAVE) {
-    v = seg_step_rows<0x142, 0xa>(v, m.s16);
-    v = seg_step_rows<0x143, 0xc>(v, m.s32);
+    // row_bcast over all rows, so the add folds into one v_add_f32_dpp; the rows that do not
+    // take the broadcast lane (row 0 and 2 for lane 15, rows 0 and 1 for lane 31) have
+    // s16 / s32 false and keep v
+    v = seg_step<0x142>(v, m.s16);
+    v = seg_step<0x143>(v, m.s32);
   }
   return v;
 }
