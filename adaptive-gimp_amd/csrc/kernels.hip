@@ -1082,8 +1082,11 @@ constexpr int P2G_MAX_CH = 2048; // largest work chunk (particles of one leaf bl
 //     16-lane rows and the row tails add with LDS float atomics into one shared tile (at most
 //     4 lanes of one instruction hit the same LDS word);
 //  3. the tiles are summed and flushed with one float atomic per non-zero node component.
+// 5 waves per SIMD: 96 VGPRs in 3D (one 8-byte spill) instead of 112 at 4 waves; the 2D
+// kernel needs 62 either way.
 template <int D>
-__global__ void __launch_bounds__(P2G_T) k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
+__global__ void __launch_bounds__(P2G_T) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
   int* perm = sd.S.perm;
