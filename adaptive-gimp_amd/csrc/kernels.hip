@@ -248,9 +248,10 @@ __global__ void k_clear_sgrid(const SceneDesc* __restrict__ Dsc) {
 
 // before bucketing, in one launch: zero the grid of the previous grid-phase lists (read
 // here, so the list counters are zeroed later, by the bucketing pass), the block counters
-// and the frozen counter; restore the octree's valid-node block flags; count the substep
+// and the frozen counter; restore the octree's valid-node block flags (or, right after a
+// rebuild that left them unsaved, save them: the same copy the other way); count the substep
 template <int D>
-__global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc) {
+__global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc, int save) {
   AGIMP_SCENE
   size_t total = 0;
   for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
@@ -272,8 +273,11 @@ __global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc) {
   fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
   fill_zero((uint32_t*)sd.S.frozen, 1);
   uint32_t* a = (uint32_t*)T.lv[0].sflag;
-  const uint32_t* v = sd.vflag;
-  for (size_t i = t0; i < sd.w_flags; i += st) a[i] = v[i];
+  uint32_t* v = sd.vflag;
+  if (save)
+    for (size_t i = t0; i < sd.w_flags; i += st) v[i] = a[i];
+  else
+    for (size_t i = t0; i < sd.w_flags; i += st) a[i] = v[i];
   if (t0 == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
 }
 
@@ -1826,7 +1830,7 @@ static inline dim3 grid2(size_t x, int nscenes) {
   return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(x, 1u << 20)), (unsigned)nscenes);
 }
 
-void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn) {
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn, bool save_flags) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 rg = grid2(std::min<size_t>(blocks_for(m.max_total_blocks, 256), (size_t)m.fill_grid), m.nscenes);
@@ -1850,13 +1854,13 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
     hipLaunchKernelGGL(k_node_A<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_mark_T<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_mark_G<DD>, lg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 1);
+    if (save_flags) hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 1);
     if (learn)
       for (int l = m.maxL - 2; l >= 0; --l) hipLaunchKernelGGL(k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);
   });
 }
 
-void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool save_flags) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
@@ -1864,7 +1868,7 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par) {
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
   const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {
-    hipLaunchKernelGGL(k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_bucket_prep<DD>, fg, dim3(256), 0, s, d, save_flags ? 1 : 0);
     hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
     hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);
