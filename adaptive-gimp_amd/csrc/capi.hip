@@ -245,7 +245,7 @@ hipError_t capture_substep(hipStream_t s, const SceneDesc* dd, const Dims& m, in
   cap.release();
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  launch_refine(s, dd, m, par, nullptr, false);  // the bucketing pass saves the block flags
+  launch_refine(s, dd, m, par, nullptr, true);  // chained: fewer clears and flag copies
   launch_bucket(s, dd, m, par, true);
   launch_p2g(s, dd, m, 0.f, par);
   launch_grid(s, dd, m, 0.f);

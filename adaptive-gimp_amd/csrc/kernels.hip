@@ -136,15 +136,41 @@ __device__ __forceinline__ void fill_zero(uint32_t* p, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += st) p[i] = 0u;
 }
 
+// zero the grid of every block in the grid-phase lists (counts and lists as left by the
+// previous substep); grid-stride over all listed nodes
+template <int D>
+__device__ __forceinline__ void clear_list_grid(const Topo& T) {
+  size_t total = 0;
+  for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
+  const size_t st = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += st) {
+    size_t acc = 0;
+    for (int l = 0; l < T.L; ++l) {
+      const size_t cnt = (size_t)T.scount[l] * Pow4<D>::v;
+      if (t < acc + cnt) {
+        const size_t q = t - acc;
+        const int b = T.lv[l].slist[q / Pow4<D>::v];
+        T.lv[l].grid[(size_t)b * Pow4<D>::v + q % Pow4<D>::v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        break;
+      }
+      acc += cnt;
+    }
+  }
+}
+
 // Before refine: cell masks, node mask, block flags, list counters.  Only flagged blocks can
 // hold marks: every cell write flags its leaf block (cflag), and every node mark lies in a
 // finest-level block whose grid-phase flag (sflag) is set (the node marks set it, it is saved
 // after refine and restored, never cleared, by bucketing).  One thread per global block
 // zeroes the block's cells if cflag is set and, for finest blocks with sflag set, its nodes;
 // then both flags.  A thread reads and clears only its own block's flags.
+// chained (the bucketing pass follows in the same graph): also zero the grid of the previous
+// grid-phase lists (k_clear_sgrid's work) and leave their counters, which the bucketing pass
+// then neither re-reads for its clear nor needs; the level map resets them.
 template <int D>
-__global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc) {
+__global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc, int chained) {
   AGIMP_SCENE
+  if (chained) clear_list_grid<D>(T);
   constexpr int WPB = Pow4<D>::v / 4;  // 32-bit words per leaf block of byte masks
   const int nb = T.total_blocks, fbase = T.lv[T.L - 1].gbase;
   uint8_t AGIMP_GLOBAL* sflag = T.lv[0].sflag;
@@ -165,7 +191,7 @@ __global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc) {
       sflag[g] = 0;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
+  if (!chained && blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
   if (blockIdx.x == 0 && threadIdx.x < 4) T.ccount[threadIdx.x] = 0;
 }
 
@@ -228,53 +254,25 @@ __global__ void __launch_bounds__(COMPACT_T) k_compact_cells(const SceneDesc* __
 template <int D>
 __global__ void k_clear_sgrid(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
-  size_t total = 0;
-  for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
-  size_t st = (size_t)gridDim.x * blockDim.x;
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += st) {
-    size_t acc = 0;
-    for (int l = 0; l < T.L; ++l) {
-      size_t cnt = (size_t)T.scount[l] * Pow4<D>::v;
-      if (t < acc + cnt) {
-        size_t q = t - acc;
-        int b = T.lv[l].slist[q / Pow4<D>::v];
-        T.lv[l].grid[(size_t)b * Pow4<D>::v + q % Pow4<D>::v] = make_float4(0.f, 0.f, 0.f, 0.f);
-        break;
-      }
-      acc += cnt;
-    }
-  }
+  clear_list_grid<D>(T);
 }
 
 // before bucketing, in one launch: zero the grid of the previous grid-phase lists (read
 // here, so the list counters are zeroed later, by the bucketing pass), the block counters
-// and the frozen counter; restore the octree's valid-node block flags (or, right after a
-// rebuild that left them unsaved, save them: the same copy the other way); count the substep
+// and the frozen counter; restore the octree's valid-node block flags; count the substep.
+// Chained to a rebuild (same graph, k_clear_refine chained): the grid is already clear and
+// the flags are saved here instead (the same copy the other way).
 template <int D>
-__global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc, int save) {
+__global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc, int chained) {
   AGIMP_SCENE
-  size_t total = 0;
-  for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
   const size_t st = (size_t)gridDim.x * blockDim.x;
   const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  for (size_t t = t0; t < total; t += st) {
-    size_t acc = 0;
-    for (int l = 0; l < T.L; ++l) {
-      size_t cnt = (size_t)T.scount[l] * Pow4<D>::v;
-      if (t < acc + cnt) {
-        size_t q = t - acc;
-        int b = T.lv[l].slist[q / Pow4<D>::v];
-        T.lv[l].grid[(size_t)b * Pow4<D>::v + q % Pow4<D>::v] = make_float4(0.f, 0.f, 0.f, 0.f);
-        break;
-      }
-      acc += cnt;
-    }
-  }
+  if (!chained) clear_list_grid<D>(T);
   fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
   fill_zero((uint32_t*)sd.S.frozen, 1);
   uint32_t* a = (uint32_t*)T.lv[0].sflag;
   uint32_t* v = sd.vflag;
-  if (save)
+  if (chained)
     for (size_t i = t0; i < sd.w_flags; i += st) v[i] = a[i];
   else
     for (size_t i = t0; i < sd.w_flags; i += st) a[i] = v[i];
@@ -1830,15 +1828,15 @@ static inline dim3 grid2(size_t x, int nscenes) {
   return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(x, 1u << 20)), (unsigned)nscenes);
 }
 
-void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn, bool save_flags) {
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn, bool chained) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 rg = grid2(std::min<size_t>(blocks_for(m.max_total_blocks, 256), (size_t)m.fill_grid), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
   const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), (size_t)m.fill_grid), m.nscenes);
   AGIMP_DIM(m.dim, {
-    hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_clear_refine<DD>, rg, dim3(256), 0, s, d);
+    if (!chained) hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
+    hipLaunchKernelGGL(k_clear_refine<DD>, rg, dim3(256), 0, s, d, chained ? 1 : 0);
     if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
     if (m.any_dynamic) {
       hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
@@ -1854,13 +1852,13 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
     hipLaunchKernelGGL(k_node_A<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_mark_T<DD>, lg, dim3(256), 0, s, d);
     hipLaunchKernelGGL(k_mark_G<DD>, lg, dim3(256), 0, s, d);
-    if (save_flags) hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 1);
+    if (!chained) hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 1);
     if (learn)
       for (int l = m.maxL - 2; l >= 0; --l) hipLaunchKernelGGL(k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);
   });
 }
 
-void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool save_flags) {
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
@@ -1868,7 +1866,7 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bo
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
   const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {
-    hipLaunchKernelGGL(k_bucket_prep<DD>, fg, dim3(256), 0, s, d, save_flags ? 1 : 0);
+    hipLaunchKernelGGL(k_bucket_prep<DD>, fg, dim3(256), 0, s, d, chained ? 1 : 0);
     hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
     hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);

@@ -26,12 +26,13 @@ size_t cells_total(const Topo& T, int lmin, int lmax);
 // par: index of the current particle buffer of every scene of the launch (G2P writes par ^ 1)
 // learn (optional, single scene): cells of the predicted units are activated after the
 // particles' own requests, before the culling
-// save_flags: the rebuild saves the octree's grid-phase block flags (k_copy_flags) for later
-// bucketing passes to restore.  A bucketing pass that directly follows a rebuild which left
-// them unsaved (save_flags false there) saves them instead of restoring them.
+// chained: a rebuild whose bucketing pass follows directly in the same stream (both passed
+// chained).  The rebuild's clear then also zeroes the previous grid-phase lists' grid, and the
+// bucketing pass saves the octree's grid-phase block flags, which an unchained rebuild saves
+// itself (k_copy_flags) and an unchained bucketing pass restores.
 void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn = nullptr,
-                   bool save_flags = true);
-void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool save_flags = false);
+                   bool chained = false);
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained = false);
 void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt);
 void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
