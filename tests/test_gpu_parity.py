@@ -396,6 +396,38 @@ def test_static_many_substeps_track_oracle(pkg, O):
     sim.close()
 
 
+@pytest.mark.parametrize("name", ["ref2d_dynamic", "small3d_plastic"])
+def test_graph_replays_mixed_with_eager_track_oracle(pkg, O, name):
+    """Graph-replayed substeps (advance: rebuild and bucketing captured as one chain, whose
+    bucketing pass saves the octree's block flags) interleaved with fixed-topology substeps,
+    which restore those flags, and with eager substeps: particles within the multi-substep
+    bars of the oracle, then octrees rebuilt from the GPU's snapshot bit for bit."""
+    dim, L, c, st, prm, gi, dt = scene(pkg, name)
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    sim.advance(dt, 3)
+    for _ in range(3):
+        ora.substep(dt)
+    for _ in range(2):
+        sim.substep_fixed_topology(dt)
+        ora.substep_fixed_topology(dt)
+    sim.substep(dt)
+    ora.substep(dt)
+    sim.advance(dt, 2)
+    for _ in range(2):
+        ora.substep(dt)
+    e = particle_errors(sim, ora)
+    assert e["fl"] == 0 and e["x"] <= 1e-5 and e["v"] <= 1e-3 and e["F"] <= 1e-4, e
+    snap = dict(x=sim.x_p.to_numpy(), v=sim.v_p.to_numpy(), F=sim.F_p.to_numpy(), C=sim.C_p.to_numpy(),
+                g=sim.g_p.to_numpy())
+    o2 = O.OracleSim(dim, L, c, snap["x"].shape[0], fp64=False,
+                     **{k: (int(v) if k == "plastic" else v) for k, v in prm.items()})
+    o2.set_particles(snap["x"], snap["v"], snap["F"], snap["C"], np.asarray(st["m"], np.float32), snap["g"])
+    sim.refine()
+    o2.refine()
+    assert_topology_equal(sim, o2, O, L)
+    sim.close()
+
+
 def test_batch_matches_single_scenes(pkg, O):
     """Batch: three 2D scenes of different sizes (two dynamic, one static) advanced together
     for 6 substeps equal the same scenes advanced one by one: octrees bit for bit, particles to
