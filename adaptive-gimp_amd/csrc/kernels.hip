@@ -1104,6 +1104,9 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
   // 3D: one node tile per wavefront and one tail per cell and wavefront, so a tail adds its
   // sums with a plain read-add-write of the node's float4 (LDS operations of a wavefront run
   // in order; tails of one instruction belong to different cells, hence different nodes).
+  // The next node's read must stay after this node's write, as another lane's node may be
+  // the same: the compiled code keeps the 27 read/write pairs in program order (the node
+  // loop's DPP scans sit between them); a wavefront fence to force it costs 6 us.
   // 2D: one tile, row-local tails, LDS float atomics (the extra scan steps cost more there
   // than the 3 atomics per node they would save).
   constexpr bool WAVE = (D == 3);
