@@ -1106,7 +1106,8 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
   // in order; tails of one instruction belong to different cells, hence different nodes).
   // The next node's read must stay after this node's write, as another lane's node may be
   // the same: the compiled code keeps the 27 read/write pairs in program order (the node
-  // loop's DPP scans sit between them); a wavefront fence to force it costs 6 us.
+  // loop's DPP scans sit between them; tests/test_p2g_isa_cpu.py checks the assembly); a
+  // wavefront fence to force it costs 6 us.
   // 2D: one tile, row-local tails, LDS float atomics (the extra scan steps cost more there
   // than the 3 atomics per node they would save).
   constexpr bool WAVE = (D == 3);
