@@ -1590,9 +1590,16 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
   int pos = blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= n) return;
   int k = perm[pos];
+  // the position is loaded beside the block id, not after the frozen test; the frozen copy
+  // uses it too, so it stays ahead of the branch
+  float x[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) x[a] = Pin.f[(size_t)(PL::X + a) * Pin.stride + k];
   int g = gb[k];
   if (g < 0) {  // frozen (fl_p = -1, adaptive_gimp.py:211-216): carried over unchanged
-    for (int c = 0; c < PL::NF; ++c) Pout.f[(size_t)c * Pout.stride + pos] = Pin.f[(size_t)c * Pin.stride + k];
+#pragma unroll
+    for (int a = 0; a < D; ++a) Pout.f[(size_t)(PL::X + a) * Pout.stride + pos] = x[a];
+    for (int c = PL::X + D; c < PL::NF; ++c) Pout.f[(size_t)c * Pout.stride + pos] = Pin.f[(size_t)c * Pin.stride + k];
     Pout.g[pos] = Pin.g[k];
     Pout.id[pos] = Pin.id[k];
     flv[pos] = -1;
@@ -1600,11 +1607,10 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
   }
   int l = level_of_gbid(T, g);
   const LevelDesc Ll = T.lv[l];
-  float x[3], w[3][3], dw[3][3];
+  float w[3][3], dw[3][3];
   int J[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    x[a] = Pin.f[(size_t)(PL::X + a) * Pin.stride + k];
     float qx = scaled(x[a], Ll, T.pow2);
     int b = (int)(qx + 0.5f) - 1;
     bspline(qx - (float)(b + 1), w[a], dw[a]);
