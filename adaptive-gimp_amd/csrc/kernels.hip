@@ -289,13 +289,18 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
   const int n = sd.n;
   unsigned long long* cnt = sd.cnt;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  // the position is loaded beside the request level, not after the test on it
+  float x[3] = {0.f, 0.f, 0.f};
+  if (k < n)
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
   const int g = k < n ? P.g[k] : -1;
   const bool live = g >= 0 && g < T.L;
   const LevelDesc Lg = T.lv[live ? g : 0];
   int b[3] = {0, 0, 0};
   if (live)
 #pragma unroll
-    for (int a = 0; a < D; ++a) b[a] = base_cell(P.f[(size_t)a * P.stride + k], Lg, T.pow2);
+    for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Lg, T.pow2);
   // particles arrive sorted by cell: a lane whose left neighbour requests the same cells
   // (same level and base cell) leaves the stores to it; out-of-range cells are still counted
   // per particle.  Every lane reaches the shuffle.
