@@ -364,8 +364,8 @@ int agimp_config_init(agimp_config* c, int32_t dim, int32_t level, int32_t coars
   c->theta_s = 7.5e-3f;
   c->hardening = 0.0f;
   c->slab_axis = 1;
-  // measured best on the 3D 1M and 4M scenes (2048) and the 2D 200k band (1024)
-  c->particles_per_chunk = (dim == 3) ? 2048 : 1024;
+  // measured best on the 3D 1M and 4M scenes and on the 2D 200k band
+  c->particles_per_chunk = 1024;
   c->device = 0;
   return 0;
 }

@@ -144,7 +144,7 @@ class AdaptiveGIMP:
         cfg.theta_c, cfg.theta_s, cfg.hardening = theta_c, theta_s, hardening
         if slab is not None:
             cfg.slab_enable, cfg.slab_axis, cfg.slab_lo, cfg.slab_hi = 1, int(slab[0]), float(slab[1]), float(slab[2])
-        if particles_per_chunk is not None:  # else the library default (2048 in 3D, 1024 in 2D)
+        if particles_per_chunk is not None:  # else the library default (1024)
             cfg.particles_per_chunk = int(particles_per_chunk)
         cfg.device = int(device)
         self._cfg = cfg

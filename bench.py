@@ -243,8 +243,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true", help="autotune the P2G chunk size first")
     ap.add_argument("--ppc", type=int, default=0,
-                    help="P2G particles per chunk (single-scene workloads; 0: the library default, 2048 in 3D, "
-                         "1024 in 2D)")
+                    help="P2G particles per chunk (single-scene workloads; 0: the library default, 1024)")
     ap.add_argument("--mode", default="both", choices=["plain", "learned", "both"],
                     help="single-scene workloads: time the plain path, the learned path (predicted octrees, "
                          "adaptive-gimp_amd/refine_mlp.npz), or both (value = the plain path)")

@@ -59,7 +59,7 @@ typedef struct agimp_config {
   int32_t slab_axis;
   float slab_lo, slab_hi;
   int32_t particles_per_chunk; /* P2G work chunk (particles of one leaf block), AGIMP_MIN/MAX_CHUNK;
-                                 agimp_config_init: 2048 in 3D, 1024 in 2D */
+                                 agimp_config_init: 1024 */
   int32_t device;          /* HIP device ordinal */
 } agimp_config;
 
