@@ -385,3 +385,175 @@ def test_c1_as_specified_is_unstable(O, pkg):
     assert one < 1e-8
     assert np.abs(ten32 - ten64).max() > 0.1 * np.abs(ten64).max()
     assert np.abs(ten64).max() > 100 * np.abs(st["v"]).max() + 1.0  # blow-up from rest
+
+
+# ------------------------------------------------------------------------- boundary conditions, plasticity
+# (extensions the workloads need: sticky walls, the slab collider, the singular-value clamp and
+# hardening; the reference has slip walls only, adaptive_gimp.py:284-287, and no plasticity)
+
+from closed_forms import (BC_CASES, STATIC_CASES, bc_single_particle, check_static_case,  # noqa: E402
+                          clamp_closed_form, kirchhoff_hardened, random_rotation, static_case_levels,
+                          stress_from_grid)
+
+
+def _single_particle_oracle(O, dim, L, c, x, v, F, Jp, dt, **prm):
+    ora = O.OracleSim(dim, L, c, 1, fp64=True, **prm)
+    ora.set_particles(np.asarray(x, float)[None], np.asarray(v, float)[None], np.asarray(F, float)[None],
+                      np.zeros((1, dim, dim)), np.ones(1), np.array([L - 1], np.int32), Jp=np.array([Jp]))
+    ora.substep(dt)
+    return ora
+
+
+@pytest.mark.parametrize("L,c", [(1, 16), (3, 4)])
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("case", BC_CASES, ids=[cs[0] for cs in BC_CASES])
+def test_walls_and_slab_closed_form(O, case, dim, L, c):
+    """One finest-level particle next to a wall band or the slab: the oracle's v', x', C' equal
+    the closed form (nodes in the band or slab hold zero velocity, slip zeroes only the normal
+    component).  A slab applied on another axis, open at its ends, or sticky walls that keep the
+    tangential velocity all fail it; a particle whose stencil lies wholly inside ends at rest."""
+    _, pos, vel, mode, slab = case
+    fs = c * 2 ** (L - 1)
+    dx = 1.0 / fs
+    x = np.array(pos[:dim]) * dx
+    v = np.array(vel[:dim])
+    grav = (0.0, -9.8, 0.0)[:dim]
+    dt = 1e-3
+    prm = dict(E=1e3, gravity=grav, wall_mode=1 if mode == "sticky" else 0)
+    if slab is not None:
+        prm["slab"] = slab
+    ora = _single_particle_oracle(O, dim, L, c, x, v, np.eye(dim), 1.0, dt, **prm)
+    r = ora.get_particles()
+    xe, ve, Ce = bc_single_particle(x, v, dt, dx, fs, grav, wall_mode=mode, slab=slab)
+    assert r["fl"][0] == L - 1
+    np.testing.assert_allclose(r["v"][0], ve, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(r["x"][0], xe, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(r["C"][0], Ce, rtol=0, atol=1e-12)
+    if not ve.any():
+        assert np.array_equal(r["x"][0], x)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_clamp_closed_form(O, dim):
+    """The singular-value clamp with J_p bookkeeping (Stomakhin et al. 2013) in the oracle's
+    G2P: with E = 0, v = 0 and no gravity the trial F is the stored F exactly."""
+    rng = np.random.default_rng(21)
+    tc, ts = 2.5e-2, 7.5e-3
+    for k in range(12):
+        s = rng.uniform(0.95, 1.03, dim)
+        if k == 0:
+            s[:] = 1.0 + 0.5 * ts  # inside the band: unchanged
+        F = random_rotation(dim, rng) @ np.diag(s) @ random_rotation(dim, rng).T
+        Jp = rng.uniform(0.9, 1.1)
+        ora = _single_particle_oracle(O, dim, 1, 16, np.full(dim, 0.5 + 0.013), np.zeros(dim), F, Jp, 1e-4,
+                                      E=0.0, gravity=(0.0,) * dim, plastic=1, theta_c=tc, theta_s=ts, hardening=10.0)
+        r = ora.get_particles()
+        Fe, Je = clamp_closed_form(F, Jp, tc, ts)
+        np.testing.assert_allclose(r["F"][0], Fe, rtol=0, atol=1e-13)
+        assert r["Jp"][0] == pytest.approx(Je, rel=1e-13)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hardened_stress_closed_form(O, dim):
+    """P2G's stress term with hardening: S recovered from the node momenta by the B-spline
+    moment identity equals -dt V_p/dx * tau(F) with mu, lambda scaled by exp(xi (1 - J_p))."""
+    rng = np.random.default_rng(22)
+    c, dt, pv, E, nu, xi = 16, 1e-4, 2e-3, 1e4, 0.3, 10.0
+    dx = 1.0 / c
+    for Jp in (1.0, 0.97, 1.02):
+        F = random_rotation(dim, rng) @ np.diag(rng.uniform(0.99, 1.01, dim)) @ random_rotation(dim, rng).T
+        x = 0.5 + rng.uniform(-0.4, 0.4, dim) * dx
+        ora = _single_particle_oracle(O, dim, 1, c, x, np.zeros(dim), F, Jp, dt, E=E, nu=nu, gravity=(0.0,) * dim,
+                                      p_vol=pv, plastic=1, theta_c=0.5, theta_s=0.5, hardening=xi)
+        _, m, v = ora.get_level(0)
+        S = stress_from_grid(m, v, x, dx)
+        Se = -float(np.float32(dt)) * pv / dx * kirchhoff_hardened(F, E, nu, Jp, xi)
+        np.testing.assert_allclose(S, Se, rtol=1e-9, atol=1e-12 * np.abs(Se).max())
+
+
+@pytest.mark.parametrize("case", STATIC_CASES, ids=[c["name"] for c in STATIC_CASES])
+def test_static_coarse_levels_against_hand_derived_masks(O, case):
+    """The same hand-derived cases through the oracle (expectations in tests/closed_forms.py)."""
+    L, c = case["L"], case["c"]
+    x = np.array([[0.49, 0.5]])
+    ora = O.OracleSim(2, L, c, 1, fp64=True)
+    ora.set_particles(x, np.zeros((1, 2)), np.eye(2)[None], np.zeros((1, 2, 2)), np.ones(1), np.zeros(1, np.int32))
+    ret, act = _static_maps(2, L, c, lambda I: static_case_levels(case, I))
+    ora.set_static_map(ret, act)
+    ora.refine()
+    check_static_case(case, [ora.get_level(l)[0] for l in range(L)], ora.get_node_mask(), ora.get_particles()["fl"][0])
+
+
+def _static_maps(dim, L, c, level_of):
+    """ret/act level arrays over the padded finest box for a map that activates, in every
+    finest cell I, the cell of level level_of(I) containing it (the init.py mask2/mask3
+    protocol: activate_cell(l, I) then return l, adaptive_gimp.py:68-70)."""
+    e = O_extents(L, c)[-1]
+    E = e + 8
+    I = np.stack(np.meshgrid(*[np.arange(-4, e + 4)] * dim, indexing="ij"), -1)
+    lev = level_of(I).astype(np.int8)
+    return lev, lev.copy()
+
+
+def O_extents(L, c):
+    return [4 * ((((c << l) // 4 + 1) + 3) & ~3) for l in range(L)]
+
+
+def test_static_coarse_levels_hand_derived(O):
+    """2D, L=2, c=4 (finest 8): the map returns level 0 left of finest x-index 4, level 1
+    elsewhere.  One particle at (0.49, 0.5): finest base cell f = (3, 3).
+    1. The 16 calls on f + {-1..2}^2 activate level-0 cells (1,1), (1,2) (I // 2 for
+       I_x in {2,3}) and level-1 cells (4..5, 2..5) (I_x in {4,5}).
+    2. GI(f) = 0, so the base cell at level 0 is (1,1); the calls on ((1..2, 1..2) * 2)
+       activate (1,1), (1,2) at level 0 and (4,2), (4,4) at level 1: nothing new.
+    The level-1 cells (4, 2..5) face level-0 cells (1,1), (1,2) across x = 4: their -x
+    neighbours (3, 2..5) become GHOST, and the face nodes (4,3), (4,5) (odd, off the level-0
+    lattice) T_JUNCTION.  fl_p = 1: the particle's level-1 cells (3..4, 3..4) hold the activated
+    (4,3) and (4,4)."""
+    x = np.array([[0.49, 0.5]])
+    ora = O.OracleSim(2, 2, 4, 1, fp64=True)
+    ora.set_particles(x, np.zeros((1, 2)), np.eye(2)[None], np.zeros((1, 2, 2)), np.ones(1), np.zeros(1, np.int32))
+    ret, act = _static_maps(2, 2, 4, lambda I: np.where(I[..., 0] < 4, 0, 1))
+    ora.set_static_map(ret, act)
+    ora.refine()
+    c0, c1 = ora.get_level(0)[0], ora.get_level(1)[0]
+    A, G, T = 1, 2, 4
+    assert sorted(map(tuple, np.argwhere(c0 == A))) == [(1, 1), (1, 2)]
+    assert not (c0 & ~A).any()
+    assert sorted(map(tuple, np.argwhere(c1 == A))) == [(i, j) for i in (4, 5) for j in (2, 3, 4, 5)]
+    assert sorted(map(tuple, np.argwhere(c1 == G))) == [(3, j) for j in (2, 3, 4, 5)]
+    assert set(np.unique(c1)) == {0, A, G}
+    nm = ora.get_node_mask()
+    assert sorted(map(tuple, np.argwhere(nm & T))) == [(4, 3), (4, 5)]
+    assert ora.get_particles()["fl"][0] == 1
+
+
+def test_static_three_levels_hand_derived(O):
+    """2D, L=3, c=2 (finest 8): the map returns level 0 left of finest x-index 4, level 2
+    elsewhere (a shift of 2 between them).  One particle at (0.49, 0.5): f = (3, 3).
+    1. Calls on f + {-1..2}^2: I_x in {2,3} -> level-0 cells I // 4 = (0,0), (0,1);
+       I_x in {4,5} -> level-2 cells (4..5, 2..5).
+    2. GI(f) = 0; the level-0 base cell is (0,0); calls on (0..1, 0..1) * 4 add level-0
+       (0,0), (0,1) and level-2 (4,0), (4,4).
+    Ghosts: each level-2 cell (4,j), j in {0,2,3,4,5}, sees level-0 cell (0, j // 4) across
+    x = 4 (l_p = 0), so level-2 cells (3,j) and their level-1 parents (1, j // 2) =
+    (1,0), (1,1), (1,2) become GHOST.  T: face nodes (4, y) with y not a multiple of 4 on those
+    faces: y in {1, 2, 3, 5, 6}.  fl_p = 2 (level-2 cells (4,3), (4,4) are among its cells)."""
+    x = np.array([[0.49, 0.5]])
+    ora = O.OracleSim(2, 3, 2, 1, fp64=True)
+    ora.set_particles(x, np.zeros((1, 2)), np.eye(2)[None], np.zeros((1, 2, 2)), np.ones(1), np.zeros(1, np.int32))
+    ret, act = _static_maps(2, 3, 2, lambda I: np.where(I[..., 0] < 4, 0, 2))
+    ora.set_static_map(ret, act)
+    ora.refine()
+    c = [ora.get_level(l)[0] for l in range(3)]
+    A, G, T = 1, 2, 4
+    assert sorted(map(tuple, np.argwhere(c[0] == A))) == [(0, 0), (0, 1)]
+    assert not c[0][c[0] != A].any()
+    assert not (c[1] == A).any()
+    assert sorted(map(tuple, np.argwhere(c[1] == G))) == [(1, 0), (1, 1), (1, 2)]
+    want2 = [(4, 0)] + [(i, j) for i in (4, 5) for j in (2, 3, 4, 5)]
+    assert sorted(map(tuple, np.argwhere(c[2] == A))) == sorted(want2)
+    assert sorted(map(tuple, np.argwhere(c[2] == G))) == [(3, j) for j in (0, 2, 3, 4, 5)]
+    nm = ora.get_node_mask()
+    assert sorted(map(tuple, np.argwhere(nm & T))) == [(4, y) for y in (1, 2, 3, 5, 6)]
+    assert ora.get_particles()["fl"][0] == 2
