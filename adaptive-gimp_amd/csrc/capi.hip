@@ -53,6 +53,19 @@ struct agimp_handle {
   std::vector<hipEvent_t> event_pool;
   double phase_ms[AGIMP_NPHASES] = {};
   int64_t phase_calls[AGIMP_NPHASES] = {};
+  // per-kernel timing: events around every launch while timing is on
+  struct KMark { int kernel; hipEvent_t a, b; };
+  struct KStat { double total_ms = 0.0, min_ms = 0.0, max_ms = 0.0; int64_t calls = 0; };
+  std::vector<KMark> kmarks;
+  std::vector<std::string> knames;  // kernel index -> launcher's kernel name
+  std::vector<KStat> kstats;
+  struct Timer : KernelTimer {
+    agimp_handle* h = nullptr;
+    int kernel = -1;
+    hipEvent_t a = nullptr;
+    void begin(const char* name, hipStream_t s) override;
+    void end(hipStream_t s) override;
+  } ktimer;
   double mean_mass = 1.0;     // mean particle mass (reference density of the unit features)
   // learned refinement prediction (agimp_learned_*)
   int learn_on = 0;
@@ -172,9 +185,9 @@ struct PhaseScope {
   }
 };
 
-// fold recorded event pairs into the per-phase totals (synchronises the stream)
+// fold recorded event pairs into the per-phase and per-kernel totals (synchronises the stream)
 int collect_marks(agimp_handle* h) {
-  if (h->marks.empty()) return 0;
+  if (h->marks.empty() && h->kmarks.empty()) return 0;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   for (auto& m : h->marks) {
     float ms = 0.f;
@@ -185,8 +198,60 @@ int collect_marks(agimp_handle* h) {
     h->event_pool.push_back(m.b);
   }
   h->marks.clear();
+  for (auto& m : h->kmarks) {
+    float ms = 0.f;
+    HIPCHK(h, hipEventElapsedTime(&ms, m.a, m.b));
+    agimp_handle::KStat& st = h->kstats[m.kernel];
+    st.min_ms = st.calls ? std::min(st.min_ms, (double)ms) : (double)ms;
+    st.max_ms = st.calls ? std::max(st.max_ms, (double)ms) : (double)ms;
+    st.total_ms += ms;
+    st.calls += 1;
+    h->event_pool.push_back(m.a);
+    h->event_pool.push_back(m.b);
+  }
+  h->kmarks.clear();
   return 0;
 }
+
+// installs the handle's kernel timer on this thread for the scope (timing on, no capture)
+struct KernelTimerScope {
+  bool on;
+  explicit KernelTimerScope(agimp_handle* h) : on(h->timing != 0) {
+    if (on) {
+      h->ktimer.h = h;
+      set_kernel_timer(&h->ktimer);
+    }
+  }
+  ~KernelTimerScope() {
+    if (on) set_kernel_timer(nullptr);
+  }
+};
+
+}  // namespace
+
+void agimp_handle::Timer::begin(const char* name, hipStream_t s) {
+  kernel = -1;
+  for (size_t i = 0; i < h->knames.size(); ++i)
+    if (h->knames[i] == name) { kernel = (int)i; break; }
+  if (kernel < 0) {
+    kernel = (int)h->knames.size();
+    h->knames.push_back(name);
+    h->kstats.emplace_back();
+  }
+  a = take_event(h);
+  if (a) (void)hipEventRecord(a, s);
+}
+
+void agimp_handle::Timer::end(hipStream_t s) {
+  if (!a) return;
+  hipEvent_t b = take_event(h);
+  if (!b) return;
+  (void)hipEventRecord(b, s);
+  h->kmarks.push_back({kernel, a, b});
+  a = nullptr;
+}
+
+namespace {
 
 Dims single_dims(const agimp_handle* h) {
   Dims m{};
@@ -207,6 +272,7 @@ Dims single_dims(const agimp_handle* h) {
 }
 
 int do_refine(agimp_handle* h) {
+  KernelTimerScope kt(h);
   PhaseScope ps(h, AGIMP_PHASE_REFINE);
   launch_refine(h->stream, h->dd, h->dims, h->host_parity);
   h->topo_valid = 1;
@@ -215,6 +281,7 @@ int do_refine(agimp_handle* h) {
 }
 
 int do_physics(agimp_handle* h, float dt) {
+  KernelTimerScope kt(h);
   {
     PhaseScope ps(h, AGIMP_PHASE_BUCKET);
     launch_bucket(h->stream, h->dd, h->dims, h->host_parity);
@@ -233,7 +300,7 @@ int do_physics(agimp_handle* h, float dt) {
   }
   h->host_parity ^= 1;
   HIPCHK(h, hipGetLastError());
-  if (h->marks.size() > 4096) {
+  if (h->marks.size() + h->kmarks.size() > 4096) {
     int rc = collect_marks(h);
     if (rc) return rc;
   }
@@ -292,6 +359,7 @@ int learn_ready(agimp_handle* h) {
 
 // candidates' features of the current particles (device only)
 int learn_features(agimp_handle* h, float dt) {
+  KernelTimerScope kt(h);
   h->U.win = (float)h->window * dt;
   launch_unit_features(h->stream, h->dd, h->D, h->n, h->host_parity, h->U);
   HIPCHK(h, hipGetLastError());
@@ -311,6 +379,7 @@ int learn_rebuild(agimp_handle* h, float dt) {
   PhaseScope ps(h, AGIMP_PHASE_REFINE);
   int rc = learn_features(h, dt);
   if (rc) return rc;
+  KernelTimerScope kt(h);
   launch_unit_predict(h->stream, h->U, h->L, nullptr);
   launch_refine(h->stream, h->dd, h->dims, h->host_parity, &h->U);
   h->topo_valid = 1;
@@ -493,6 +562,8 @@ int agimp_destroy(agimp_handle* h) {
   h->fixed[0].release();
   h->fixed[1].release();
   for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
+  for (auto& m : h->kmarks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
+  if (h->ktimer.a) (void)hipEventDestroy(h->ktimer.a);
   for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
   for (void* p : h->allocs) (void)hipFree(p);
   if (h->hd.st_ret) (void)hipFree((void*)h->hd.st_ret);
@@ -730,6 +801,32 @@ int agimp_get_timings(agimp_handle* h, double* ms, int64_t* calls, int32_t n) {
   return 0;
 }
 
+int agimp_get_kernel_timings(agimp_handle* h, char* names, int32_t name_len, double* total_ms, int64_t* calls,
+                             double* min_ms, double* max_ms, int32_t cap, int32_t* count) {
+  if (!h || cap < 0 || !count || (names && name_len < 1)) return AGIMP_EINVAL;
+  int rc = collect_marks(h);
+  if (rc) return rc;
+  int k = 0;
+  for (size_t i = 0; i < h->kstats.size(); ++i) {
+    agimp_handle::KStat& st = h->kstats[i];
+    if (!st.calls) continue;
+    if (k < cap) {
+      if (names) {
+        std::strncpy(names + (size_t)k * name_len, h->knames[i].c_str(), (size_t)name_len - 1);
+        names[(size_t)k * name_len + name_len - 1] = 0;
+      }
+      if (total_ms) total_ms[k] = st.total_ms;
+      if (calls) calls[k] = st.calls;
+      if (min_ms) min_ms[k] = st.min_ms;
+      if (max_ms) max_ms[k] = st.max_ms;
+    }
+    ++k;
+    st = agimp_handle::KStat{};
+  }
+  *count = k;
+  return 0;
+}
+
 int agimp_level_extent(agimp_handle* h, int32_t l) {
   if (!h || l < 0 || l >= h->L) return AGIMP_EINVAL;
   return h->hd.T.lv[l].e;
@@ -963,25 +1060,18 @@ int agimp_learned_reset_stats(agimp_handle* h) {
 
 // ------------------------------------------------------------------------------ batches
 
-int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out) {
-  if (!scenes || n < 1 || !out) return AGIMP_EINVAL;
-  *out = nullptr;
-  agimp_batch* b = new agimp_batch();
+// launch geometry of a batch from its members' current settings (a member that switches
+// between static and dynamic adaptivity after the batch was made changes it)
+static Dims batch_dims(const std::vector<agimp_handle*>& scenes) {
+  const int n = (int)scenes.size();
   Dims m{};
-  m.dim = scenes[0] ? scenes[0]->D : 0;
+  m.dim = scenes[0]->D;
   m.nscenes = n;
   m.list_grid = std::max(16, 2048 / n);
   m.fill_grid = std::max(16, 4096 / n);  // the grid-stride loops absorb the rest
   m.grid_grid = std::max(16, 4096 / n);
   int p2g = 0;
-  for (int i = 0; i < n; ++i) {
-    agimp_handle* h = scenes[i];
-    if (!h || h->D != m.dim || h->cfg.device != scenes[0]->cfg.device || !h->has_particles) {
-      delete b;
-      return AGIMP_EINVAL;
-    }
-    (void)hipStreamSynchronize(h->stream);
-    b->scenes.push_back(h);
+  for (const agimp_handle* h : scenes) {
     m.maxL = std::max(m.maxL, h->L);
     m.any_single |= h->L == 1;
     m.any_static |= h->hd.static_mode;
@@ -992,7 +1082,31 @@ int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out
     p2g = std::max(p2g, h->max_chunks);
   }
   m.p2g_grid = std::max(16, std::min(p2g, 8192 / n));
-  b->dims = m;
+  return m;
+}
+
+static bool same_dims(const Dims& a, const Dims& b) {
+  return a.dim == b.dim && a.nscenes == b.nscenes && a.maxL == b.maxL && a.any_single == b.any_single &&
+         a.any_static == b.any_static && a.any_dynamic == b.any_dynamic && a.max_n == b.max_n &&
+         a.max_total_blocks == b.max_total_blocks && a.max_fill_words == b.max_fill_words &&
+         a.list_grid == b.list_grid && a.fill_grid == b.fill_grid && a.p2g_grid == b.p2g_grid &&
+         a.grid_grid == b.grid_grid;
+}
+
+int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out) {
+  if (!scenes || n < 1 || !out) return AGIMP_EINVAL;
+  *out = nullptr;
+  agimp_batch* b = new agimp_batch();
+  for (int i = 0; i < n; ++i) {
+    agimp_handle* h = scenes[i];
+    if (!h || !scenes[0] || h->D != scenes[0]->D || h->cfg.device != scenes[0]->cfg.device || !h->has_particles) {
+      delete b;
+      return AGIMP_EINVAL;
+    }
+    (void)hipStreamSynchronize(h->stream);
+    b->scenes.push_back(h);
+  }
+  b->dims = batch_dims(b->scenes);
   b->device = scenes[0]->cfg.device;
   if (hipSetDevice(b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&b->dd, sizeof(SceneDesc) * (size_t)n) != hipSuccess) {
@@ -1010,7 +1124,8 @@ int agimp_batch_create(agimp_handle* const* scenes, int32_t n, agimp_batch** out
 }
 
 // make every scene's batch descriptor list its current buffer at index b->par (the scene may
-// have been advanced on its own since the last batch call); uploads entries that changed
+// have been advanced on its own since the last batch call); uploads entries that changed, in
+// order on the batch stream (b->host keeps the sources alive until the stream is synchronised)
 static int batch_sync_buffers(agimp_batch* b) {
   for (size_t i = 0; i < b->scenes.size(); ++i) {
     agimp_handle* h = b->scenes[i];
@@ -1019,7 +1134,8 @@ static int batch_sync_buffers(agimp_batch* b) {
     want.P[b->par ^ 1] = h->hd.P[h->host_parity ^ 1];
     if (std::memcmp(&want, &b->host[i], sizeof(SceneDesc)) == 0) continue;
     b->host[i] = want;
-    if (hipMemcpy(b->dd + i, &want, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess) return AGIMP_EHIP;
+    if (hipMemcpyAsync(b->dd + i, &b->host[i], sizeof(SceneDesc), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+      return AGIMP_EHIP;
   }
   return 0;
 }
@@ -1049,13 +1165,29 @@ int agimp_batch_set_particles_per_chunk(agimp_batch* b, const int32_t* ch) {
 int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps) {
   if (!b || !dt || steps < 0) return AGIMP_EINVAL;
   const int n = (int)b->scenes.size();
+  // work a member queued on its own stream (agimp_advance and agimp_substep return before it
+  // ends) must finish before the batch touches the same buffers
+  for (agimp_handle* h : b->scenes)
+    if (hipStreamSynchronize(h->stream) != hipSuccess) {
+      b->err = "batch_advance: a member's stream failed";
+      return AGIMP_EHIP;
+    }
+  // a member switched between static and dynamic adaptivity since the graphs were captured:
+  // their kernel set no longer fits, so recapture
+  const Dims m = batch_dims(b->scenes);
+  if (!same_dims(m, b->dims)) {
+    b->dims = m;
+    b->graph[0].release();
+    b->graph[1].release();
+  }
   // per-scene dt in the descriptors (read by the kernels at run time)
   for (int i = 0; i < n; ++i) {
     agimp_handle* h = b->scenes[i];
     if (h->hd.ph.dt != dt[i]) {
       h->hd.ph.dt = dt[i];
       h->graph_dt = -1.f;
-      if (hipMemcpy(h->dd, &h->hd, sizeof(SceneDesc), hipMemcpyHostToDevice) != hipSuccess) return AGIMP_EHIP;
+      if (hipMemcpyAsync(h->dd, &h->hd, sizeof(SceneDesc), hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        return AGIMP_EHIP;
     }
   }
   if (batch_sync_buffers(b)) return AGIMP_EHIP;

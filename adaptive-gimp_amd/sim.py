@@ -172,15 +172,23 @@ class AdaptiveGIMP:
         self.node_mask = NodeMaskField(self)
 
         particle_initializer(self)                                # :64
-        self.grid_initializer = grid_initializer                  # :65
-        self.static_adaptivity = grid_initializer is not None     # :66
+        self.set_grid_initializer(grid_initializer)               # :65-66
+        self._push()
+
+    def set_grid_initializer(self, grid_initializer):
+        """Static adaptivity from ``grid_initializer`` (adaptive_gimp.py:65-66, :72-85), or
+        dynamic adaptivity with None.  Takes effect at the next rebuild."""
+        self.grid_initializer = grid_initializer
+        self.static_adaptivity = grid_initializer is not None
         if self.static_adaptivity:
-            ret, act = static_level_maps(grid_initializer, dim, level, coarsest_size)
+            ret, act = static_level_maps(grid_initializer, self.dim, self.level, self.coarsest_size)
             self._static_maps = (np.ascontiguousarray(ret), np.ascontiguousarray(act))
             _lib.check(self._L.agimp_set_static_levels(self._h, _lib.ptr(self._static_maps[0], ctypes.c_int8),
                                                        _lib.ptr(self._static_maps[1], ctypes.c_int8)),
                        self._h, "agimp_set_static_levels")
-        self._push()
+        else:
+            self._static_maps = None
+            _lib.check(self._L.agimp_set_static_levels(self._h, None, None), self._h, "agimp_set_static_levels")
 
     # ------------------------------------------------------------------ lifecycle
     def close(self):
@@ -289,18 +297,48 @@ class AdaptiveGIMP:
                                              len(_lib.PHASES)), self._h, "agimp_get_timings")
         return {p: (float(ms[i]), int(calls[i])) for i, p in enumerate(_lib.PHASES)}
 
+    def reset_counters(self):
+        _lib.check(self._L.agimp_reset_counters(self._h), self._h, "agimp_reset_counters")
+
     def counters(self):
         out = np.zeros(8, np.int64)
         _lib.check(self._L.agimp_get_counters(self._h, _lib.ptr(out, ctypes.c_int64), 8), self._h, "get_counters")
         return {"oob": int(out[0]), "frozen": int(out[1]), "active": int(out[2]), "chunks": int(out[3]),
                 "nonfinite": int(out[4]), "criterion_failures": int(out[5])}
 
+    def kernel_timings(self):
+        """{kernel: (total_ms, launches, min_ms, max_ms)} since the last call (synchronises);
+        recorded while ``set_timing(True)`` is on, eager substeps only."""
+        cap, nl = 128, 64
+        names = ctypes.create_string_buffer(cap * nl)
+        tot, mn, mx = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        calls = np.zeros(cap, np.int64)
+        cnt = ctypes.c_int32(0)
+        dp = lambda a: _lib.ptr(a, ctypes.c_double)
+        _lib.check(self._L.agimp_get_kernel_timings(self._h, names, nl, dp(tot), _lib.ptr(calls, ctypes.c_int64), dp(mn),
+                                                    dp(mx), cap, ctypes.byref(cnt)), self._h, "agimp_get_kernel_timings")
+        raw = names.raw
+        out = {}
+        for k in range(min(cnt.value, cap)):
+            nm = raw[k * nl:(k + 1) * nl].split(b"\0", 1)[0].decode()
+            out[nm] = (float(tot[k]), int(calls[k]), float(mn[k]), float(mx[k]))
+        return out
+
     def profile_report(self):
-        """Per-phase GPU time since the last call, as text: the counterpart of the Taichi kernel
-        profiler report that run_3d.py:12 and :43 print.  Phases are timed with HIP events on
-        the library's stream while ``set_timing(True)`` is on."""
+        """GPU time per kernel and per phase since the last call, as text: the counterpart of
+        the Taichi kernel profiler report that run_3d.py:12 and :43 print (share of the total,
+        total, launches, min / avg / max per launch, kernel name).  Kernels and phases are timed
+        with HIP events on the library's stream while ``set_timing(True)`` is on."""
+        kt = self.kernel_timings()
         t = self.timings()
-        lines = [f"{'phase':<10} {'calls':>7} {'total ms':>11} {'avg ms':>9}"]
+        ktot = sum(v[0] for v in kt.values())
+        lines = [f"{'%':>6} {'total ms':>10} {'launches':>8} {'min ms':>9} {'avg ms':>9} {'max ms':>9}  kernel"]
+        for name, (ms, calls, mn, mx) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+            pct = 100.0 * ms / ktot if ktot > 0 else 0.0
+            lines.append(f"{pct:6.2f} {ms:10.3f} {calls:8d} {mn:9.4f} {ms / calls:9.4f} {mx:9.4f}  {name}")
+        lines.append(f"{100.0 if ktot > 0 else 0.0:6.2f} {ktot:10.3f} {'':>8} {'':>9} {'':>9} {'':>9}  [all kernels]")
+        lines.append("")
+        lines.append(f"{'phase':<10} {'calls':>7} {'total ms':>11} {'avg ms':>9}")
         for name, (ms, calls) in t.items():
             if calls:
                 lines.append(f"{name:<10} {calls:>7d} {ms:>11.3f} {ms / calls:>9.4f}")
@@ -475,13 +513,21 @@ class Batch:
             raise _lib.AgimpError(f"agimp_batch_create failed with code {rc}")
         self._b = b
 
+    def _push_members(self):
+        # host-side edits of a member (from_numpy, item assignment) made after the batch was
+        # created must reach the device before the batch advances the same buffers
+        for s in self.sims:
+            s._push()
+
     def set_particles_per_chunk(self, ch):
+        self._push_members()
         ch = np.ascontiguousarray(np.broadcast_to(np.asarray(ch, np.int32), (len(self.sims),)))
         rc = self._L.agimp_batch_set_particles_per_chunk(self._b, _lib.ptr(ch, ctypes.c_int32))
         if rc != 0:
             raise _lib.AgimpError(f"agimp_batch_set_particles_per_chunk failed with code {rc}")
 
     def advance(self, dt, steps):
+        self._push_members()
         dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (len(self.sims),)))
         rc = self._L.agimp_batch_advance(self._b, _lib.ptr(dt, ctypes.c_float), int(steps))
         if rc != 0:
