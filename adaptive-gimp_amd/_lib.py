@@ -37,7 +37,8 @@ EXPORTS = [
     "agimp_config_init", "agimp_create", "agimp_destroy", "agimp_set_particles", "agimp_set_static_levels",
     "agimp_refine", "agimp_substep", "agimp_substep_fixed_topology", "agimp_advance", "agimp_get_particles",
     "agimp_get_level", "agimp_get_node_mask", "agimp_get_counters", "agimp_reset_counters",
-    "agimp_level_extent", "agimp_set_timing", "agimp_get_timings", "agimp_sync", "agimp_last_error",
+    "agimp_level_extent", "agimp_set_timing", "agimp_get_timings", "agimp_get_kernel_timings", "agimp_sync",
+    "agimp_last_error",
     "agimp_set_particles_per_chunk", "agimp_batch_create", "agimp_batch_destroy",
     "agimp_batch_set_particles_per_chunk", "agimp_batch_advance", "agimp_batch_last_error",
     "agimp_host_svd", "agimp_host_constitutive",
@@ -93,6 +94,8 @@ def load():
         "agimp_level_extent": [vp, ctypes.c_int32],
         "agimp_set_timing": [vp, ctypes.c_int32],
         "agimp_get_timings": [vp, dp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32],
+        "agimp_get_kernel_timings": [vp, ctypes.c_char_p, ctypes.c_int32, dp, ctypes.POINTER(ctypes.c_int64), dp, dp,
+                                     ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
         "agimp_sync": [vp],
         "agimp_last_error": [vp],
         "agimp_set_particles_per_chunk": [vp, ctypes.c_int32],

@@ -1827,6 +1827,10 @@ __global__ void k_dense_node(const SceneDesc* __restrict__ Dsc, int* out) {
 }
 
 // ------------------------------------------------------------------------------ launchers
+static thread_local KernelTimer* g_kernel_timer = nullptr;
+void set_kernel_timer(KernelTimer* t) { g_kernel_timer = t; }
+KernelTimer* kernel_timer() { return g_kernel_timer; }
+
 static inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
 #define AGIMP_DIM(D, ...) \
@@ -1850,26 +1854,26 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
   const dim3 lg = grid2(m.list_grid, m.nscenes);
   const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), (size_t)m.fill_grid), m.nscenes);
   AGIMP_DIM(m.dim, {
-    if (!chained) hipLaunchKernelGGL(k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_clear_refine<DD>, rg, dim3(256), 0, s, d, chained ? 1 : 0);
-    if (m.any_static) hipLaunchKernelGGL(k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
+    if (!chained) AGIMP_LAUNCH("k_clear_sgrid", k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
+    AGIMP_LAUNCH("k_clear_refine", k_clear_refine<DD>, rg, dim3(256), 0, s, d, chained ? 1 : 0);
+    if (m.any_static) AGIMP_LAUNCH("k_activate_static", k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
     if (m.any_dynamic) {
-      hipLaunchKernelGGL(k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
+      AGIMP_LAUNCH("k_activate_dynamic", k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
       if (learn) launch_unit_activate(s, d, m.dim, (int)m.max_n, par, *learn);
       if (m.maxL > 1) {
-        hipLaunchKernelGGL(k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 0);
-        hipLaunchKernelGGL(k_mark_hier<DD>, lg, dim3(256), 0, s, d);
-        hipLaunchKernelGGL(k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 1);
-        hipLaunchKernelGGL(k_split<DD>, lg, dim3(256), 0, s, d);
+        AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 0);
+        AGIMP_LAUNCH("k_mark_hier", k_mark_hier<DD>, lg, dim3(256), 0, s, d);
+        AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 1);
+        AGIMP_LAUNCH("k_split", k_split<DD>, lg, dim3(256), 0, s, d);
       }
     }
-    hipLaunchKernelGGL(k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 2);
-    hipLaunchKernelGGL(k_node_A<DD>, lg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_mark_T<DD>, lg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_mark_G<DD>, lg, dim3(256), 0, s, d);
-    if (!chained) hipLaunchKernelGGL(k_copy_flags, fg, dim3(256), 0, s, d, 1);
+    AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 2);
+    AGIMP_LAUNCH("k_node_A", k_node_A<DD>, lg, dim3(256), 0, s, d);
+    AGIMP_LAUNCH("k_mark_T", k_mark_T<DD>, lg, dim3(256), 0, s, d);
+    AGIMP_LAUNCH("k_mark_G", k_mark_G<DD>, lg, dim3(256), 0, s, d);
+    if (!chained) AGIMP_LAUNCH("k_copy_flags", k_copy_flags, fg, dim3(256), 0, s, d, 1);
     if (learn)
-      for (int l = m.maxL - 2; l >= 0; --l) hipLaunchKernelGGL(k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);
+      for (int l = m.maxL - 2; l >= 0; --l) AGIMP_LAUNCH("k_cover", k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);
   });
 }
 
@@ -1881,60 +1885,60 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bo
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
   const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {
-    hipLaunchKernelGGL(k_bucket_prep<DD>, fg, dim3(256), 0, s, d, chained ? 1 : 0);
-    hipLaunchKernelGGL(k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
-    hipLaunchKernelGGL(k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
-    hipLaunchKernelGGL(k_scan_final, tg, dim3(SCAN_T), 0, s, d);
-    hipLaunchKernelGGL(k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);
-    hipLaunchKernelGGL(k_scompact<DD>, cg, dim3(COMPACT_T), 0, s, d);
+    AGIMP_LAUNCH("k_bucket_prep", k_bucket_prep<DD>, fg, dim3(256), 0, s, d, chained ? 1 : 0);
+    AGIMP_LAUNCH("k_levelmap", k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
+    AGIMP_LAUNCH("k_scan_reduce", k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
+    AGIMP_LAUNCH("k_scan_final", k_scan_final, tg, dim3(SCAN_T), 0, s, d);
+    AGIMP_LAUNCH("k_scatter_blocks", k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);
+    AGIMP_LAUNCH("k_scompact", k_scompact<DD>, cg, dim3(COMPACT_T), 0, s, d);
   });
 }
 
 void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par) {
-  AGIMP_DIM(m.dim, hipLaunchKernelGGL(k_p2g<DD>, grid2(m.p2g_grid, m.nscenes), dim3(P2G_T), 0, s, d, dt, par));
+  AGIMP_DIM(m.dim, AGIMP_LAUNCH("k_p2g", k_p2g<DD>, grid2(m.p2g_grid, m.nscenes), dim3(P2G_T), 0, s, d, dt, par));
 }
 
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
   const dim3 gg = grid2(m.grid_grid, m.nscenes);
   AGIMP_DIM(m.dim, {
-    for (int l = m.maxL - 1; l >= 1; --l) hipLaunchKernelGGL(k_restrict<DD>, gg, dim3(256), 0, s, d, l, dt);
-    if (m.any_single) hipLaunchKernelGGL(k_gridop<DD>, gg, dim3(256), 0, s, d, dt);
-    for (int l = 1; l < m.maxL; ++l) hipLaunchKernelGGL(k_prolong<DD>, gg, dim3(256), 0, s, d, l, dt);
+    for (int l = m.maxL - 1; l >= 1; --l) AGIMP_LAUNCH("k_restrict", k_restrict<DD>, gg, dim3(256), 0, s, d, l, dt);
+    if (m.any_single) AGIMP_LAUNCH("k_gridop", k_gridop<DD>, gg, dim3(256), 0, s, d, dt);
+    for (int l = 1; l < m.maxL; ++l) AGIMP_LAUNCH("k_prolong", k_prolong<DD>, gg, dim3(256), 0, s, d, l, dt);
   });
 }
 
 void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par) {
   AGIMP_DIM(m.dim,
-            hipLaunchKernelGGL(k_g2p<DD>, grid2(blocks_for(m.max_n, 256), m.nscenes), dim3(256), 0, s, d, dt, par));
+            AGIMP_LAUNCH("k_g2p", k_g2p<DD>, grid2(blocks_for(m.max_n, 256), m.nscenes), dim3(256), 0, s, d, dt, par));
 }
 
 void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n, int par) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_stress<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par));
+  AGIMP_DIM(dim, AGIMP_LAUNCH("k_stress", k_stress<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par));
 }
 
 void launch_upload(hipStream_t s, const SceneDesc* d, int par, int dim, int n, const float* x, const float* v,
                    const float* F, const float* C, const float* m, const int* g, const float* Jp) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_upload<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par, x, v, F, C, m,
+  AGIMP_DIM(dim, AGIMP_LAUNCH("k_upload", k_upload<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par, x, v, F, C, m,
                                     g, Jp));
 }
 
 void launch_download(hipStream_t s, const SceneDesc* d, int par, int dim, int n, float* x, float* v, float* F,
                      float* C, float* Jp, int* fl, int* g) {
-  AGIMP_DIM(dim, hipLaunchKernelGGL(k_download<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par, x, v, F, C,
+  AGIMP_DIM(dim, AGIMP_LAUNCH("k_download", k_download<DD>, grid2(blocks_for(n, 256), 1), dim3(256), 0, s, d, par, x, v, F, C,
                                     Jp, fl, g));
 }
 
 void launch_dense_level(hipStream_t s, const SceneDesc* d, const Topo& T, int l, int* cell, float* m, float* v) {
   size_t tot = 1;
   for (int a = 0; a < T.dim; ++a) tot *= (size_t)T.lv[l].e;
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_dense_level<DD>, grid2(blocks_for(tot, 256), 1), dim3(256), 0, s, d, l, cell,
+  AGIMP_DIM(T.dim, AGIMP_LAUNCH("k_dense_level", k_dense_level<DD>, grid2(blocks_for(tot, 256), 1), dim3(256), 0, s, d, l, cell,
                                       m, v));
 }
 
 void launch_dense_node(hipStream_t s, const SceneDesc* d, const Topo& T, int* out) {
   size_t tot = 1;
   for (int a = 0; a < T.dim; ++a) tot *= (size_t)T.lv[T.L - 1].e;
-  AGIMP_DIM(T.dim, hipLaunchKernelGGL(k_dense_node<DD>, grid2(blocks_for(tot, 256), 1), dim3(256), 0, s, d, out));
+  AGIMP_DIM(T.dim, AGIMP_LAUNCH("k_dense_node", k_dense_node<DD>, grid2(blocks_for(tot, 256), 1), dim3(256), 0, s, d, out));
 }
 
 }  // namespace agimp

@@ -22,6 +22,25 @@ struct Dims {
   int grid_grid;            // workgroups per scene for the grid passes
 };
 
+// Per-kernel timing, the counterpart of the Taichi kernel profiler (run_3d.py:12, :43): while a
+// timer is installed on the calling thread, the launchers bracket every kernel launch with
+// begin/end on the launch stream.  Never installed during graph capture.
+struct KernelTimer {
+  virtual void begin(const char* name, hipStream_t s) = 0;
+  virtual void end(hipStream_t s) = 0;
+  virtual ~KernelTimer() = default;
+};
+void set_kernel_timer(KernelTimer* t);  // per thread; nullptr switches it off
+KernelTimer* kernel_timer();
+
+#define AGIMP_LAUNCH(name, kern, grid, block, shmem, stream, ...)             \
+  do {                                                                         \
+    ::agimp::KernelTimer* kt_ = ::agimp::kernel_timer();                       \
+    if (kt_) kt_->begin(name, stream);                                         \
+    hipLaunchKernelGGL(kern, grid, block, shmem, stream, __VA_ARGS__);         \
+    if (kt_) kt_->end(stream);                                                 \
+  } while (0)
+
 size_t cells_total(const Topo& T, int lmin, int lmax);
 // par: index of the current particle buffer of every scene of the launch (G2P writes par ^ 1)
 // learn (optional, single scene): cells of the predicted units are activated after the

@@ -592,39 +592,39 @@ static inline unsigned nblk(int64_t n, unsigned t, unsigned cap) {
 
 void launch_unit_features(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U) {
   LEARN_DIM(dim, {
-    hipLaunchKernelGGL(k_unit_clear_acc, dim3(nblk((int64_t)U.nu * UNIT_ACC, 256, 2048)), dim3(256), 0, s, U);
-    hipLaunchKernelGGL(k_unit_clear_flags, dim3(nblk(((int64_t)U.nu + 3) / 4, 256, 2048)), dim3(256), 0, s, U);
-    hipLaunchKernelGGL(k_unit_accumulate<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U);
-    hipLaunchKernelGGL(k_unit_compact, dim3(nblk(U.nu, 256, 1u << 30)), dim3(256), 0, s, U.uflag, U.nu, U.occ,
+    AGIMP_LAUNCH("k_unit_clear_acc", k_unit_clear_acc, dim3(nblk((int64_t)U.nu * UNIT_ACC, 256, 2048)), dim3(256), 0, s, U);
+    AGIMP_LAUNCH("k_unit_clear_flags", k_unit_clear_flags, dim3(nblk(((int64_t)U.nu + 3) / 4, 256, 2048)), dim3(256), 0, s, U);
+    AGIMP_LAUNCH("k_unit_accumulate", k_unit_accumulate<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U);
+    AGIMP_LAUNCH("k_unit_compact", k_unit_compact, dim3(nblk(U.nu, 256, 1u << 30)), dim3(256), 0, s, U.uflag, U.nu, U.occ,
                        U.counts + 0);
-    hipLaunchKernelGGL(k_unit_dilate<DD>, dim3(2048), dim3(256), 0, s, d, U);
-    hipLaunchKernelGGL(k_unit_compact, dim3(nblk(U.nu, 256, 1u << 30)), dim3(256), 0, s, U.cflag, U.nu, U.cand,
+    AGIMP_LAUNCH("k_unit_dilate", k_unit_dilate<DD>, dim3(2048), dim3(256), 0, s, d, U);
+    AGIMP_LAUNCH("k_unit_compact", k_unit_compact, dim3(nblk(U.nu, 256, 1u << 30)), dim3(256), 0, s, U.cflag, U.nu, U.cand,
                        U.counts + 1);
-    hipLaunchKernelGGL(k_unit_features<DD>, dim3(1024), dim3(256), 0, s, d, U);
+    AGIMP_LAUNCH("k_unit_features", k_unit_features<DD>, dim3(1024), dim3(256), 0, s, d, U);
   });
 }
 
 void launch_unit_predict(hipStream_t s, const LearnDesc& U, int L, int* cls_out) {
-  hipLaunchKernelGGL(k_unit_predict, dim3(512), dim3(PRED_T), 0, s, U, L, cls_out);
+  AGIMP_LAUNCH("k_unit_predict", k_unit_predict, dim3(512), dim3(PRED_T), 0, s, U, L, cls_out);
 }
 
 void launch_unit_activate(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U) {
   LEARN_DIM(dim, {
-    if (U.mode & 1) hipLaunchKernelGGL(k_unit_activate<DD>, dim3(2048), dim3(256), 0, s, d, U);
-    if (U.mode & 2) hipLaunchKernelGGL(k_activate_ahead<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U);
+    if (U.mode & 1) AGIMP_LAUNCH("k_unit_activate", k_unit_activate<DD>, dim3(2048), dim3(256), 0, s, d, U);
+    if (U.mode & 2) AGIMP_LAUNCH("k_activate_ahead", k_activate_ahead<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U);
   });
 }
 
 void launch_unit_labels_clear(hipStream_t s, const LearnDesc& U) {
-  hipLaunchKernelGGL(k_unit_clear_labels, dim3(nblk(U.nu, 256, 2048)), dim3(256), 0, s, U);
+  AGIMP_LAUNCH("k_unit_clear_labels", k_unit_clear_labels, dim3(nblk(U.nu, 256, 2048)), dim3(256), 0, s, U);
 }
 
 void launch_unit_labels_accumulate(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U) {
-  LEARN_DIM(dim, hipLaunchKernelGGL(k_unit_requests<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U));
+  LEARN_DIM(dim, AGIMP_LAUNCH("k_unit_requests", k_unit_requests<DD>, dim3(nblk(n, 256, 1u << 30)), dim3(256), 0, s, d, par, U));
 }
 
 void launch_unit_label_classes(hipStream_t s, const LearnDesc& U, int L, int* out) {
-  hipLaunchKernelGGL(k_unit_label_classes, dim3(1024), dim3(256), 0, s, U, L, out);
+  AGIMP_LAUNCH("k_unit_label_classes", k_unit_label_classes, dim3(1024), dim3(256), 0, s, U, L, out);
 }
 
 }  // namespace agimp

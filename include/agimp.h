@@ -130,6 +130,14 @@ int agimp_reset_counters(agimp_handle* h);
 #define AGIMP_NPHASES 5
 int agimp_set_timing(agimp_handle* h, int32_t on);
 int agimp_get_timings(agimp_handle* h, double* ms, int64_t* calls, int32_t n);
+/* Per-kernel GPU time while timing is on (the Taichi kernel profiler's per-kernel table,
+ * ti.profiler.print_kernel_profiler_info at run_3d.py:43): HIP events recorded on the handle's
+ * stream around every kernel launch of the eager path.  Synchronises, returns what accumulated
+ * since the last call and resets it.  Entry k: names + k*name_len (NUL-terminated), total ms,
+ * launches, shortest and longest launch in ms; at most cap entries are written and *count
+ * receives the number of kernels.  Any output pointer but count may be NULL. */
+int agimp_get_kernel_timings(agimp_handle* h, char* names, int32_t name_len, double* total_ms, int64_t* calls,
+                             double* min_ms, double* max_ms, int32_t cap, int32_t* count);
 
 /* Reference extent e_l of level l. */
 int agimp_level_extent(agimp_handle* h, int32_t l);
