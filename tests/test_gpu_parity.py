@@ -342,17 +342,31 @@ def test_snapshot_resume(pkg, O, tmp_path):
     b.close()
 
 
-def test_profile_report_lists_phases(pkg):
+def test_profile_report_lists_kernels_and_phases(pkg):
+    """The per-kernel report (run_3d.py:12, :43 kernel profiler): every kernel of the eager
+    substep appears with one launch per substep (P2G, G2P) or more, the kernel times add up to
+    no more than the phase times around them, and the phase table lists all five phases."""
     wl = pkg.scenes.small_3d(seed=2, n=4000, L=3, c=16)
     st = wl.state
     sim = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, st["x"].shape[0], _init_from(st), None, **wl.params)
     sim.set_timing(True)
     for _ in range(2):
         sim.substep(wl.dt)
+    kt = sim.kernel_timings()
+    ph = sim.timings()
+    for k in ("k_p2g", "k_g2p", "k_levelmap", "k_activate_dynamic", "k_mark_T", "k_mark_G", "k_restrict", "k_prolong"):
+        assert k in kt, (k, sorted(kt))
+    assert kt["k_p2g"][1] == 2 and kt["k_g2p"][1] == 2
+    assert kt["k_restrict"][1] == 2 * (wl.level - 1)
+    for tot, calls, mn, mx in kt.values():
+        assert calls >= 1 and 0 <= mn <= tot / calls <= mx + 1e-9
+    assert sum(v[0] for v in kt.values()) <= 1.05 * sum(v[0] for v in ph.values()) + 1e-3
+    assert all(ph[p][1] == 2 for p in ("refine", "bucket", "p2g", "grid", "g2p")), ph
+    sim.substep(wl.dt)
     rep = sim.profile_report()
-    rows = {ln.split()[0]: ln.split() for ln in rep.splitlines()[1:]}
-    assert "total" in rows and float(rows["total"][-1]) > 0
-    assert any(r[0] != "total" and int(r[1]) >= 2 for r in rows.values()), rep
+    lines = rep.splitlines()
+    assert any(ln.endswith(" k_p2g") for ln in lines) and any(ln.endswith(" k_g2p") for ln in lines), rep
+    assert any(ln.startswith("total") for ln in lines), rep
     sim.close()
 
 
@@ -453,6 +467,71 @@ def test_batch_matches_single_scenes(pkg, O):
         assert np.array_equal(a.fl_p.to_numpy(), s.fl_p.to_numpy())
     b.close()
     for s in batch_sims + single:
+        s.close()
+
+
+def _assert_same_run(a, s, L):
+    for l in range(L):
+        assert np.array_equal(a.cell_mask[l].to_numpy(), s.cell_mask[l].to_numpy()), l
+    xa, xs = a.x_p.to_numpy(), s.x_p.to_numpy()
+    va, vs = a.v_p.to_numpy(), s.v_p.to_numpy()
+    assert np.abs(xa - xs).max() <= 1e-6
+    assert np.abs(va - vs).max() <= 1e-4 * max(np.abs(vs).max(), 1e-30)
+    assert np.array_equal(a.fl_p.to_numpy(), s.fl_p.to_numpy())
+
+
+def test_batch_sees_member_edits_and_queued_member_work(pkg, O):
+    """A member edited after the batch was made, and a member whose own graph replays are still
+    queued when the batch advances (no synchronisation between them), end exactly where the
+    same calls on stand-alone simulators end."""
+    S = pkg.scenes
+    specs = [(S.reference_case0_2d(0, seed=4), 6000), (S.reference_case0_2d(0, seed=5), 9000)]
+    mk = lambda init, n: pkg.AdaptiveGIMP(2, 4, 16, n, init)
+    bs = [mk(*sp) for sp in specs]
+    single = [mk(*sp) for sp in specs]
+    b = pkg.Batch(bs)
+    for s in (bs[0], single[0]):
+        v = s.v_p.to_numpy()
+        v[:, 0] += 0.5
+        s.v_p.from_numpy(v)
+    bs[1].advance(2e-5, 3)  # queued on the member's stream; the batch must wait for it
+    b.advance(2e-5, 3)
+    bs[0].substep(2e-5)
+    bs[1].advance(2e-5, 2)
+    single[0].advance(2e-5, 3)
+    single[0].substep(2e-5)
+    single[1].advance(2e-5, 8)
+    for a, s in zip(bs, single):
+        _assert_same_run(a, s, 4)
+    b.close()
+    for s in bs + single:
+        s.close()
+
+
+def test_batch_follows_member_switch_to_static(pkg, O):
+    """A member switched from dynamic to static adaptivity after the batch was made: the
+    batch recaptures its graphs, so the member's octree comes from its grid initializer (its
+    particles are not left frozen) and matches the oracle's from the same snapshot."""
+    S = pkg.scenes
+    gi = S.REFERENCE_GRID_INITIALIZERS[2]
+    sims = [pkg.AdaptiveGIMP(2, 4, 16, n, S.reference_case0_2d(0, seed=sd)) for sd, n in ((6, 5000), (7, 7000))]
+    b = pkg.Batch(sims)
+    b.advance(2e-5, 1)
+    sims[1].set_grid_initializer(gi)
+    sims[1].reset_counters()
+    b.advance(2e-5, 2)
+    assert sims[1].counters()["frozen"] == 0
+    s = sims[1]
+    snap = dict(x=s.x_p.to_numpy(), v=s.v_p.to_numpy(), F=s.F_p.to_numpy(), C=s.C_p.to_numpy(), g=s.g_p.to_numpy())
+    o2 = O.OracleSim(2, 4, 16, 7000, fp64=False)
+    o2.set_particles(snap["x"], snap["v"], snap["F"], snap["C"], np.ones(7000, np.float32), snap["g"])
+    ret, act = pkg.scenes.static_level_maps(gi, 2, 4, 16)
+    o2.set_static_map(ret, act)
+    s.refine()
+    o2.refine()
+    assert_topology_equal(s, o2, O, 4)
+    b.close()
+    for s in sims:
         s.close()
 
 
