@@ -36,6 +36,7 @@ class AgimpConfig(ctypes.Structure):
 EXPORTS = [
     "agimp_config_init", "agimp_create", "agimp_destroy", "agimp_set_particles", "agimp_set_static_levels",
     "agimp_refine", "agimp_substep", "agimp_substep_fixed_topology", "agimp_advance", "agimp_get_particles",
+    "agimp_get_particles_noalloc",
     "agimp_get_level", "agimp_get_node_mask", "agimp_get_counters", "agimp_reset_counters",
     "agimp_level_extent", "agimp_set_timing", "agimp_get_timings", "agimp_get_kernel_timings", "agimp_sync",
     "agimp_last_error",
@@ -87,6 +88,7 @@ def load():
         "agimp_substep_fixed_topology": [vp, ctypes.c_float],
         "agimp_advance": [vp, ctypes.c_float, ctypes.c_int32],
         "agimp_get_particles": [vp, fp, fp, fp, fp, fp, ip, ip],
+        "agimp_get_particles_noalloc": [vp, fp, fp, fp, fp, fp, ip, ip],
         "agimp_get_level": [vp, ctypes.c_int32, ip, fp, fp],
         "agimp_get_node_mask": [vp, ip],
         "agimp_get_counters": [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32],

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -410,7 +411,66 @@ int snapshot(agimp_handle* h, int par, bool save) {
   return 0;
 }
 
+// Staging of agimp_get_particles: x, v [n][d], F, C [n][d][d], Jp [n] float32, fl, g [n] int32
+// in caller order, written by k_download, then copied to the host.
+size_t particle_staging_bytes(const agimp_handle* h) {
+  const size_t n = (size_t)h->n, nv = n * h->D, nm = n * h->D * h->D;
+  return (2 * nv + 2 * nm + n) * sizeof(float) + 2 * n * sizeof(int);
+}
+
+int download_particles(agimp_handle* h, char* staging, float* x, float* v, float* F, float* C, float* Jp,
+                       int32_t* fl, int32_t* g) {
+  const int D = h->D, n = h->n;
+  size_t nv = (size_t)n * D, nm = (size_t)n * D * D;
+  float* dx = (float*)staging;
+  float* dv = dx + nv;
+  float* dF = dv + nv;
+  float* dC = dF + nm;
+  float* dJ = dC + nm;
+  int* dfl = (int*)(dJ + n);
+  int* dg = dfl + n;
+  launch_download(h->stream, h->dd, h->host_parity, D, n, dx, dv, dF, dC, dJ, dfl, dg);
+  hipError_t e = hipGetLastError();
+  if (x && e == hipSuccess) e = hipMemcpyAsync(x, dx, nv * 4, hipMemcpyDeviceToHost, h->stream);
+  if (v && e == hipSuccess) e = hipMemcpyAsync(v, dv, nv * 4, hipMemcpyDeviceToHost, h->stream);
+  if (F && e == hipSuccess) e = hipMemcpyAsync(F, dF, nm * 4, hipMemcpyDeviceToHost, h->stream);
+  if (C && e == hipSuccess) e = hipMemcpyAsync(C, dC, nm * 4, hipMemcpyDeviceToHost, h->stream);
+  if (Jp && e == hipSuccess) e = hipMemcpyAsync(Jp, dJ, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
+  if (fl && e == hipSuccess) e = hipMemcpyAsync(fl, dfl, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
+  if (g && e == hipSuccess) e = hipMemcpyAsync(g, dg, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("get_particles: ") + hipGetErrorString(e));
+  return 0;
+}
+
 }  // namespace
+
+// Scratch of agimp_get_particles_noalloc: the staging buffer that agimp_get_particles allocates
+// on every call, held instead in one file-scope device array.  Each GPU the library runs on
+// holds its own copy, at its own address, until the process ends.  In host code the name
+// `scratch` is only a stand-in for the symbol: its device address is taken once per device
+// (agimp_create, with the device current) and only that address is used.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static std::vector<char*> g_scratch_ptr;  // per device ordinal; nullptr until set up
+
+// resolve the scratch array's address on the current device (`device`) once
+static int setup_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0) return AGIMP_EINVAL;
+  if ((size_t)device >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return AGIMP_EHIP;
+  g_scratch_ptr[device] = (char*)p;
+  return 0;
+}
+
+static char* scratch_of_device(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && (size_t)device < g_scratch_ptr.size()) ? g_scratch_ptr[device] : nullptr;
+}
 
 extern "C" {
 
@@ -456,6 +516,11 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   h->n = (int)c.n_particles;
   int rc = 0;
   if (hipSetDevice(c.device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return AGIMP_EHIP;
+  }
+  if (setup_scratch(c.device) != 0) {  // the device address of agimp_get_particles_noalloc's scratch
+    (void)hipStreamDestroy(h->stream);
     delete h;
     return AGIMP_EHIP;
   }
@@ -703,31 +768,24 @@ int agimp_advance(agimp_handle* h, float dt, int32_t n) {
 int agimp_get_particles(agimp_handle* h, float* x, float* v, float* F, float* C, float* Jp, int32_t* fl,
                         int32_t* g) {
   if (!h) return AGIMP_EINVAL;
-  const int D = h->D, n = h->n;
-  size_t nv = (size_t)n * D, nm = (size_t)n * D * D;
-  size_t bytes = (2 * nv + 2 * nm + (size_t)n) * sizeof(float) + 2 * (size_t)n * sizeof(int);
   char* tmp = nullptr;
-  HIPCHK(h, hipMalloc(&tmp, bytes));
-  float* dx = (float*)tmp;
-  float* dv = dx + nv;
-  float* dF = dv + nv;
-  float* dC = dF + nm;
-  float* dJ = dC + nm;
-  int* dfl = (int*)(dJ + n);
-  int* dg = dfl + n;
-  launch_download(h->stream, h->dd, h->host_parity, D, n, dx, dv, dF, dC, dJ, dfl, dg);
-  hipError_t e = hipGetLastError();
-  if (x && e == hipSuccess) e = hipMemcpyAsync(x, dx, nv * 4, hipMemcpyDeviceToHost, h->stream);
-  if (v && e == hipSuccess) e = hipMemcpyAsync(v, dv, nv * 4, hipMemcpyDeviceToHost, h->stream);
-  if (F && e == hipSuccess) e = hipMemcpyAsync(F, dF, nm * 4, hipMemcpyDeviceToHost, h->stream);
-  if (C && e == hipSuccess) e = hipMemcpyAsync(C, dC, nm * 4, hipMemcpyDeviceToHost, h->stream);
-  if (Jp && e == hipSuccess) e = hipMemcpyAsync(Jp, dJ, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
-  if (fl && e == hipSuccess) e = hipMemcpyAsync(fl, dfl, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
-  if (g && e == hipSuccess) e = hipMemcpyAsync(g, dg, (size_t)n * 4, hipMemcpyDeviceToHost, h->stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  HIPCHK(h, hipMalloc(&tmp, particle_staging_bytes(h)));
+  const int rc = download_particles(h, tmp, x, v, F, C, Jp, fl, g);
   (void)hipFree(tmp);
-  if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("get_particles: ") + hipGetErrorString(e));
-  return 0;
+  return rc;
+}
+
+int agimp_get_particles_noalloc(agimp_handle* h, float* x, float* v, float* F, float* C, float* Jp, int32_t* fl,
+                                int32_t* g) {
+  if (!h) return AGIMP_EINVAL;
+  const size_t bytes = particle_staging_bytes(h);
+  if (bytes > NOALLOC_SCRATCH_BYTES)
+    return fail(h, AGIMP_EINVAL, "get_particles_noalloc: " + std::to_string(h->n) + " particles need " +
+                                     std::to_string(bytes) + " bytes of staging, above the " +
+                                     std::to_string(NOALLOC_SCRATCH_BYTES) + "-byte scratch array");
+  char* p = scratch_of_device(h->cfg.device);
+  if (!p) return fail(h, AGIMP_ESTATE, "get_particles_noalloc: no scratch address for this device");
+  return download_particles(h, p, x, v, F, C, Jp, fl, g);
 }
 
 int agimp_get_level(agimp_handle* h, int32_t l, int32_t* cell_mask, float* m, float* v) {

@@ -104,6 +104,16 @@ int agimp_set_particles_per_chunk(agimp_handle* h, int32_t ch);
 int agimp_get_particles(agimp_handle* h, float* x, float* v, float* F, float* C, float* Jp, int32_t* fl,
                         int32_t* g);
 
+/* agimp_get_particles without any device allocation, for callers that cannot allocate
+ * (allocation-free serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments and results; the device staging goes through one file-scope
+ * scratch array of 64 MiB per GPU instead of a per-call allocation, which limits it to scenes
+ * of at most 64 MiB / (4 (2d + 2d^2 + 1) + 8) particles: 621,378 in 3D, 1,118,481 in 2D; larger
+ * scenes get AGIMP_EINVAL.  All calls on one GPU share the array: calls that overlap on one GPU
+ * (from several threads) race. */
+int agimp_get_particles_noalloc(agimp_handle* h, float* x, float* v, float* F, float* C, float* Jp, int32_t* fl,
+                                int32_t* g);
+
 /* Dense cell_mask[l], ad_grid_m[l], ad_grid_v[l] (adaptive_gimp.py:39-42); any may be NULL. */
 int agimp_get_level(agimp_handle* h, int32_t l, int32_t* cell_mask, float* m, float* v);
 
