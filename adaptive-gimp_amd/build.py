@@ -5,7 +5,8 @@
 Each .hip translation unit is compiled with hipcc in parallel, then linked into one
 shared library next to this file.  Packed-FP32 instructions are disabled
 (``-Xclang -target-feature -Xclang -packed-fp32-ops``); float atomics use the hardware
-``global_atomic_add_f32`` (``-munsafe-fp-atomics``).
+``global_atomic_add_f32`` (``-munsafe-fp-atomics``).  The build fails if the device code of
+the 3D P2G kernel does not keep its LDS tile updates in program order (p2g_tile_order_ok).
 """
 from __future__ import annotations
 
@@ -38,10 +39,46 @@ FLAGS = [
 ]
 
 
+def p2g_tile_order_ok(asm: str) -> tuple[bool, str]:
+    """The 3D P2G kernel adds each cell's sums into its wavefront's LDS node tile with a plain
+    read-add-write per stencil node.  That is exact only while every node's tile read stays
+    after the previous node's tile write in the compiled code (another lane's node can be the
+    same one; a wavefront's LDS operations complete in issue order).  Checks that the kernel's
+    ds_read_b128 / ds_write_b128 sequence is: the tile clear's writes, 27 read-write pairs, then
+    the 4 flush reads.  Returns (ok, the sequence)."""
+    import re
+
+    names = [n for n in re.findall(r"^(_ZN5agimp\w+):", asm, re.M) if "k_p2gILi3" in n]
+    if len(names) != 1:
+        return False, f"expected one 3D P2G kernel, found {names}"
+    start = asm.index(names[0] + ":")
+    body = asm[start:asm.index(".Lfunc_end", start)]
+    seq = "".join("R" if op == "ds_read_b128" else "W"
+                  for op in re.findall(r"^\s+(ds_read_b128|ds_write_b128)\b", body, re.M))
+    m = re.fullmatch(r"W+((?:RW)+)(R+)", seq)
+    ok = bool(m) and len(m.group(1)) == 2 * 27 and len(m.group(2)) == 4
+    return ok, seq
+
+
 def build(verbose: bool = False) -> str:
     cc = hipcc()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+
+    def p2g_order_check():
+        # the device assembly of kernels.hip with the same flags as the object that is linked
+        asm_path = os.path.join(objdir, "kernels_gfx.s")
+        cmd = [cc, *FLAGS, "-I", INCLUDE, "-x", "hip", "--cuda-device-only", "-S", "-o", asm_path,
+               os.path.join(CSRC, "kernels.hip")]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc -S failed for kernels.hip:\n{r.stderr}")
+        with open(asm_path) as f:
+            ok, seq = p2g_tile_order_ok(f.read())
+        if not ok:
+            raise RuntimeError("the compiler reordered the 3D P2G LDS tile updates (read/write sequence "
+                               f"{seq}); the plain read-add-write would race, refusing to build")
+        return asm_path
 
     def compile_one(src):
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
@@ -53,8 +90,10 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(r.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+    with ThreadPoolExecutor(max_workers=len(SOURCES) + 1) as ex:
+        order = ex.submit(p2g_order_check)
         objs = list(ex.map(compile_one, SOURCES))
+        order.result()
     tmp = OUT + ".tmp"
     cmd = [cc, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
