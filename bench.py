@@ -65,42 +65,71 @@ def _oracle_for(wl, threads, fp64=False):
     return ora
 
 
-def cpu_baseline(workloads, seconds_target=12.0):
-    """The oracle (float32 + FP64 SVD, C++ OpenMP) timed on this host's cores: per scene one
-    warm-up substep, then substeps until ~seconds_target have elapsed over all scenes (at
-    least 2 per scene).  Test infrastructure, used here only as the reported baseline."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
-    per_scene = seconds_target / len(workloads)
+def host_threads():
+    """Threads for the CPU legs: OMP_NUM_THREADS when set (the GPU box sets it to the CPU share
+    of one GPU), else every CPU this process may run on."""
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return max(1, omp or aff), {"os_cpu_count": os.cpu_count(), "affinity_cpus": aff,
+                                "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def _time_oracle(workloads, threads, per_scene_s, max_sub=50):
     work, elapsed, nsub = 0.0, 0.0, 0
     for wl in workloads:
         ora = _oracle_for(wl, threads)
-        ora.substep(wl.dt)
-        t_end = time.perf_counter() + per_scene
+        ora.substep(wl.dt)  # warm-up
+        t_end = time.perf_counter() + per_scene_s
         k = 0
         t0 = time.perf_counter()
-        while k < 2 or time.perf_counter() < t_end:
+        while k < 1 or time.perf_counter() < t_end:
             ora.substep(wl.dt)
             k += 1
-            if k >= 50:
+            if k >= max_sub:
                 break
         elapsed += time.perf_counter() - t0
         work += wl.n * k
         nsub += k
         del ora
+    return work, elapsed, nsub
+
+
+def cpu_baseline(workloads, seconds_target=12.0, seconds_1t=8.0):
+    """The oracle (float32 + FP64 SVD, C++ OpenMP) timed on this host's cores: per scene one
+    warm-up substep, then substeps until ~seconds_target have elapsed over all scenes; then the
+    same on one thread for ~seconds_1t (at least one substep per scene).  Test infrastructure,
+    used here only as the reported baseline."""
+    threads, host = host_threads()
+    work, elapsed, nsub = _time_oracle(workloads, threads, seconds_target / len(workloads))
+    w1, e1, n1 = _time_oracle(workloads[:1], 1, seconds_1t, max_sub=4)
     names = workloads[0].name if len(workloads) == 1 else f"{len(workloads)} scenes ({workloads[0].name}...)"
     return {"value": work / elapsed, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
             "sample": f"{nsub} substeps of {names} after one warm-up substep each "
-                      f"({work / 1e6:.1f} M particle-substeps in {elapsed:.1f} s; oracle C++ OpenMP "
-                      "restatement, float32 + FP64 SVD)"}
+                      f"({work / 1e6:.1f} M particle-substeps in {elapsed:.1f} s on {threads} threads; oracle C++ "
+                      "OpenMP restatement, float32 + FP64 SVD)",
+            "host": host,
+            "one_thread": {"value": w1 / e1, "substeps": n1, "seconds": round(e1, 1),
+                           "scene": workloads[0].name}}
 
+
+# kernels of each substep phase (launch names of the library's per-kernel timer)
+PHASE_KERNELS = {
+    "refine": ("k_clear_sgrid", "k_clear_refine", "k_activate_static", "k_activate_dynamic", "k_compact_cells",
+               "k_mark_hier", "k_split", "k_node_A", "k_mark_T", "k_mark_G", "k_copy_flags"),
+    "bucket": ("k_bucket_prep", "k_levelmap", "k_scan_reduce", "k_scan_final", "k_scatter_blocks", "k_scompact"),
+    "p2g": ("k_p2g",),
+    "grid": ("k_restrict", "k_gridop", "k_prolong"),
+    "g2p": ("k_g2p",),
+}
 
 METRIC = "particle-substeps/s (3D 1M p, 3 levels) plain vs learned-adapt; HBM GB/s"
 
 
 def _threads():
-    t = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    return max(1, min(t, 16))
+    return host_threads()[0]
 
 
 def _new_sim(pkg, wl, ppc):
@@ -192,11 +221,20 @@ def bench_single(pkg, wl, args, np):
     elapsed = time.perf_counter() - t0
     # per-phase breakdown: eager substeps with HIP events on the library's stream
     sim.timings()
+    sim.kernel_timings()
     sim.set_timing(True)
-    for _ in range(min(args.steps, 10)):
+    nt = min(args.steps, 10)
+    for _ in range(nt):
         sim.substep(wl.dt)
-    tim = sim.timings()
+    ktim = sim.kernel_timings()
+    sim.timings()
     sim.set_timing(False)
+    # per phase: the sum of its kernels' event-timed durations, per substep
+    tim = {ph: [0.0, nt] for ph in PHASE_KERNELS}
+    for k, (ms, calls, _, _) in ktim.items():
+        for ph, names in PHASE_KERNELS.items():
+            if k in names:
+                tim[ph][0] += ms
     n_nodes = int(sum(int(np.count_nonzero(sim.ad_grid_m[l].to_numpy())) for l in range(wl.level)))
     counters = sim.counters()
     sim.close()
@@ -205,7 +243,7 @@ def bench_single(pkg, wl, args, np):
     ref.advance(wl.dt, args.warmup + args.steps)
     plain_x = ref.x_p.to_numpy().astype(np.float64)
     ref.close()
-    return elapsed, tim, n_nodes, counters, tuned, plain_x
+    return elapsed, tim, n_nodes, counters, tuned, plain_x, ktim
 
 
 def bench_batch(pkg, wls, args):
@@ -306,10 +344,11 @@ def main():
 
             lm = import_module(pkg.__name__ + ".learned")
             model = lm.load_model()
-        elapsed, tim, n_nodes, counters, tuned, plain_x = bench_single(pkg, wl, args, np)
+        elapsed, tim, n_nodes, counters, tuned, plain_x, ktim = bench_single(pkg, wl, args, np)
         b_p2g, b_g2p = algorithmic_bytes(wl.dim, wl.n, n_nodes)
         avg = {k: (v[0] / v[1] if v[1] else float("nan")) for k, v in tim.items()}
-        kern = {"p2g": (b_p2g, avg["p2g"]), "g2p": (b_g2p, avg["g2p"])}
+        kavg = {k: v[0] / v[1] for k, v in ktim.items() if v[1]}
+        kern = {"p2g": (b_p2g, kavg["k_p2g"]), "g2p": (b_g2p, kavg["k_g2p"])}
         dom = max(kern, key=lambda k: kern[k][1])
         achieved = kern[dom][0] / (kern[dom][1] * 1e-3) / 1e9
         line.update({
@@ -323,8 +362,9 @@ def main():
                          "traffic": measured_traffic(dom) if wl.name.startswith("c3") else None,
                          "bytes_per_launch": kern[dom][0], "avg_kernel_ms": kern[dom][1], "active_nodes": n_nodes},
             "phase_ms_eager": {k: round(v, 4) for k, v in avg.items()},
-            "p2g_gbs": b_p2g / (avg["p2g"] * 1e-3) / 1e9,
-            "g2p_gbs": b_g2p / (avg["g2p"] * 1e-3) / 1e9,
+            "kernel_ms_eager": {k: round(v, 4) for k, v in sorted(kavg.items(), key=lambda kv: -kv[1])},
+            "p2g_gbs": b_p2g / (kavg["k_p2g"] * 1e-3) / 1e9,
+            "g2p_gbs": b_g2p / (kavg["k_g2p"] * 1e-3) / 1e9,
             "counters": counters,
         })
         if tuned:
@@ -335,7 +375,10 @@ def main():
             line["learned"] = {
                 "value": lv, "ms_per_step": el * 1e3 / args.steps, "speedup_vs_plain": lv / line["value"],
                 "parts": args.learned_parts, "window": args.window or model[1].get("window", 20), "tau": args.tau,
-                "stats": st, "criterion_failures_executed": 0,
+                "stats": st,
+                # particles whose check failed in substeps that were kept (the device counter; a
+                # window rolled back restores it with the particles)
+                "criterion_failures_executed": lcnt["criterion_failures"],
                 "criterion_failures_detected": st["violations"],
                 "x_dev_from_plain": {"max": float(dev.max()), "p99": float(np.percentile(dev, 99)),
                                      "after_substeps": args.warmup + args.steps},
