@@ -164,19 +164,73 @@ def bench_learned(pkg, wl, args, np, model, plain_x):
     return elapsed, stats, dev, counters
 
 
+def _worst_particles(np, wl, x, v, fl, ref, k=100):
+    """Where the largest velocity errors sit: for the k particles with the largest |v - v_ref|,
+    the share whose transfer level differs from the oracle's, whose position lies within two
+    cells (of their level) of a wall band or of the slab faces, or within 1e-3 cell of a face of
+    their cell (where a rounding difference moves the base cell); and the worst one itself."""
+    ev = np.linalg.norm(v - ref["v"], axis=1)
+    top = np.argsort(ev)[-k:][::-1]
+    fs = wl.coarsest_size * 2 ** (wl.level - 1)
+    gap = wl.params.get("boundary_gap", 4)
+    lvl = np.where(ref["fl"] >= 0, ref["fl"], wl.level - 1)
+    dxl = 1.0 / (wl.coarsest_size * 2.0 ** lvl)
+    q = x / dxl[:, None]
+    face = np.abs(q - np.round(q)).min(axis=1)  # distance to the nearest cell face, in cells
+    wall_d = np.minimum(x - gap / fs, (fs - gap) / fs - x).min(axis=1) / dxl
+    slab = wl.params.get("slab")
+    slab_d = (np.minimum(np.abs(x[:, slab[0]] - slab[1]), np.abs(x[:, slab[0]] - slab[2])) / dxl
+              if slab else np.full(x.shape[0], np.inf))
+    i = int(top[0])
+    return {
+        "k": k,
+        "share_fl_differs": float(np.mean(fl[top] != ref["fl"][top])),
+        "share_near_wall": float(np.mean(wall_d[top] < 2)),
+        "share_near_slab": float(np.mean(slab_d[top] < 2)),
+        "share_near_cell_face": float(np.mean(face[top] < 1e-3)),
+        "worst": {"index": i, "x": [float(a) for a in x[i]], "v": [float(a) for a in v[i]],
+                  "v_ref": [float(a) for a in ref["v"][i]], "fl": int(fl[i]), "fl_ref": int(ref["fl"][i]),
+                  "cells_to_wall": float(wall_d[i]), "cells_to_slab": float(slab_d[i]),
+                  "cells_to_face": float(face[i])},
+    }
+
+
+def _errors(np, wl, x, v, F, ref):
+    ex = np.linalg.norm(x - ref["x"], axis=1)
+    vs = max(float(np.abs(ref["v"]).max()), 1e-30)
+    mom = np.sum(wl.state["m"][:, None] * v, axis=0)
+    mom_ref = np.sum(wl.state["m"][:, None] * ref["v"], axis=0)
+    ev = np.linalg.norm(v - ref["v"], axis=1) / vs
+    return {"x_max_abs": float(ex.max()), "x_p99_abs": float(np.percentile(ex, 99)),
+            "v_max_rel": float(ev.max()), "v_p99_rel": float(np.percentile(ev, 99)),
+            "F_max_abs": float(np.abs(F - ref["F"]).max()),
+            "momentum_rel": float(np.linalg.norm(mom - mom_ref) / max(np.linalg.norm(mom_ref), 1e-30))}
+
+
 def accuracy(pkg, wl, args, np, model):
     """Positions and velocities of the plain and the learned GPU paths against the FP64 oracle
-    (float64 particles and grid, FP64 SVD) after `n` substeps from the same initial state."""
+    (float64 particles and grid, FP64 SVD) after `n` substeps from the same float32 initial
+    state, next to the noise floor: the float32 oracle against the same FP64 run (the same
+    algorithm, only the precision and the summation order differ)."""
     n = args.accuracy_steps
-    ora = _oracle_for(wl, _threads(), fp64=True)
-    t0 = time.perf_counter()
-    for _ in range(n):
-        ora.substep(wl.dt)
-    t_ora = time.perf_counter() - t0
-    ref = ora.get_particles()
-    mass0 = float(np.sum(wl.state["m"]))
+    st = wl.state
+    f32 = lambda a: np.asarray(a, np.float32)
+    refs = {}
+    t_ora = 0.0
+    for name, fp64 in (("fp64", True), ("fp32", False)):
+        ora = _oracle_for(wl, _threads(), fp64=fp64)
+        ora.set_particles(f32(st["x"]), f32(st["v"]), f32(st["F"]), f32(st["C"]), f32(st["m"]), st["g"])
+        t0 = time.perf_counter()
+        for _ in range(n):
+            ora.substep(wl.dt)
+        t_ora += time.perf_counter() - t0
+        refs[name] = ora.get_particles()
+        del ora
+    ref = refs["fp64"]
     out = {"substeps": n, "reference": "oracle FP64 (C++ restatement, double particles and grid)",
-           "oracle_seconds": round(t_ora, 1)}
+           "oracle_seconds": round(t_ora, 1),
+           "noise_floor_fp32_oracle": _errors(np, wl, refs["fp32"]["x"], refs["fp32"]["v"], refs["fp32"]["F"], ref)}
+    out["noise_floor_fp32_oracle"]["fl_differs"] = int(np.sum(refs["fp32"]["fl"] != ref["fl"]))
     runs = [("plain", None)] + ([("learned", model)] if model is not None else [])
     for name, mdl in runs:
         if mdl is None:
@@ -189,17 +243,14 @@ def accuracy(pkg, wl, args, np, model):
         x = sim.x_p.to_numpy().astype(np.float64)
         v = sim.v_p.to_numpy().astype(np.float64)
         F = sim.F_p.to_numpy().astype(np.float64)
+        fl = sim.fl_p.to_numpy()
         sim.close()
-        ex = np.linalg.norm(x - ref["x"], axis=1)
-        vs = max(float(np.abs(ref["v"]).max()), 1e-30)
-        mom = np.sum(wl.state["m"][:, None] * v, axis=0)
-        mom_ref = np.sum(wl.state["m"][:, None] * ref["v"], axis=0)
-        ev = np.linalg.norm(v - ref["v"], axis=1) / vs
-        out[name] = {"x_max_abs": float(ex.max()), "x_p99_abs": float(np.percentile(ex, 99)),
-                     "v_max_rel": float(ev.max()), "v_p99_rel": float(np.percentile(ev, 99)),
-                     "F_max_abs": float(np.abs(F - ref["F"]).max()),
-                     "momentum_rel": float(np.linalg.norm(mom - mom_ref) / max(np.linalg.norm(mom_ref), 1e-30)),
-                     "mass": "exact (particle masses are constant)" if mass0 > 0 else None}
+        out[name] = _errors(np, wl, x, v, F, ref)
+        out[name]["fl_differs"] = int(np.sum(fl != ref["fl"]))
+        out[name]["mass"] = "exact (particle masses are constant)"
+        out[name]["largest_v_errors"] = _worst_particles(np, wl, x, v, fl, ref)
+    out["noise_floor_fp32_oracle"]["largest_v_errors"] = _worst_particles(np, wl, refs["fp32"]["x"], refs["fp32"]["v"],
+                                                                         refs["fp32"]["fl"], ref)
     return out
 
 
