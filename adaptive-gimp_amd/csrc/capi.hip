@@ -45,6 +45,7 @@ struct agimp_handle {
   int max_chunks = 0;
   float graph_dt = -1.f;      // dt stored in the device descriptor for graph replays
   Captured graph[2];          // one substep starting from buffer 0 / buffer 1
+  Captured chain[2];          // SUBSTEP_CHAIN substeps in one graph, from buffer 0 / buffer 1
   std::vector<void*> allocs;
   std::string err;
   // phase timing with HIP events on the handle's stream (agimp_set_timing)
@@ -91,6 +92,7 @@ struct agimp_batch {
   SceneDesc* dd = nullptr;
   Dims dims{};
   Captured graph[2];
+  Captured chain[2];
   std::string err;
 };
 
@@ -323,6 +325,56 @@ hipError_t capture_substep(hipStream_t s, const SceneDesc* dd, const Dims& m, in
   if (le != hipSuccess) return le;
   if (e != hipSuccess) return e;
   return hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
+}
+
+// SUBSTEP_CHAIN full substeps in one graph, starting from buffer `par` (the buffer index flips
+// after each substep, so an even count ends on `par` again): one host launch advances them all
+constexpr int SUBSTEP_CHAIN = 8;
+hipError_t capture_chain(hipStream_t s, const SceneDesc* dd, const Dims& m, int par, Captured& cap) {
+  cap.release();
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  for (int i = 0; i < SUBSTEP_CHAIN; ++i) {
+    const int p = par ^ (i & 1);
+    launch_refine(s, dd, m, p, nullptr, true);
+    launch_bucket(s, dd, m, p, true);
+    launch_p2g(s, dd, m, 0.f, p);
+    launch_grid(s, dd, m, 0.f);
+    launch_g2p(s, dd, m, 0.f, p);
+  }
+  hipError_t le = hipGetLastError();
+  e = hipStreamEndCapture(s, &cap.graph);
+  if (le != hipSuccess) return le;
+  if (e != hipSuccess) return e;
+  return hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
+}
+
+// replay n substeps starting from buffer *par: whole chains while at least SUBSTEP_CHAIN remain,
+// then single-substep graphs; captures what is missing on first use
+hipError_t replay_substeps(hipStream_t s, const SceneDesc* dd, const Dims& m, Captured* one, Captured* chain,
+                           int* par, int n) {
+  static_assert(SUBSTEP_CHAIN % 2 == 0, "a chain must end on the buffer it starts from");
+  for (int p = 0; p < 2; ++p) {
+    if (!one[p].exec) {
+      hipError_t e = capture_substep(s, dd, m, p, one[p]);
+      if (e != hipSuccess) { one[p].release(); return e; }
+    }
+  }
+  if (n >= SUBSTEP_CHAIN && !chain[*par].exec) {
+    hipError_t e = capture_chain(s, dd, m, *par, chain[*par]);
+    if (e != hipSuccess) { chain[*par].release(); return e; }
+  }
+  int i = 0;
+  for (; i + SUBSTEP_CHAIN <= n; i += SUBSTEP_CHAIN) {
+    hipError_t e = hipGraphLaunch(chain[*par].exec, s);
+    if (e != hipSuccess) return e;
+  }
+  for (; i < n; ++i) {
+    hipError_t e = hipGraphLaunch(one[*par].exec, s);
+    if (e != hipSuccess) return e;
+    *par ^= 1;
+  }
+  return hipSuccess;
 }
 
 // substep without the rebuild (bucketing + physics) of one scene, for reused octrees
@@ -624,6 +676,8 @@ int agimp_destroy(agimp_handle* h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   h->graph[0].release();
   h->graph[1].release();
+  h->chain[0].release();
+  h->chain[1].release();
   h->fixed[0].release();
   h->fixed[1].release();
   for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
@@ -682,6 +736,8 @@ int agimp_set_static_levels(agimp_handle* h, const int8_t* ret_level, const int8
   if (!h) return AGIMP_EINVAL;
   h->graph[0].release();
   h->graph[1].release();
+  h->chain[0].release();
+  h->chain[1].release();
   if (!ret_level || !act_level) {
     h->hd.static_mode = 0;
     h->dims = single_dims(h);
@@ -746,21 +802,12 @@ int agimp_advance(agimp_handle* h, float dt, int32_t n) {
     }
     return 0;
   }
-  // replay one captured substep n times; dt lives in the device descriptor
+  // replay captured substeps (chains of SUBSTEP_CHAIN, then single ones); dt lives in the
+  // device descriptor
   rc = set_graph_dt(h, dt);
   if (rc) return rc;
-  for (int p = 0; p < 2; ++p) {
-    if (h->graph[p].exec) continue;
-    hipError_t e = capture_substep(h->stream, h->dd, h->dims, p, h->graph[p]);
-    if (e != hipSuccess) {
-      h->graph[p].release();
-      return fail(h, AGIMP_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
-    }
-  }
-  for (int i = 0; i < n; ++i) {
-    HIPCHK(h, hipGraphLaunch(h->graph[h->host_parity].exec, h->stream));
-    h->host_parity ^= 1;
-  }
+  hipError_t e = replay_substeps(h->stream, h->dd, h->dims, h->graph, h->chain, &h->host_parity, n);
+  if (e != hipSuccess) return fail(h, AGIMP_EHIP, std::string("graph replay: ") + hipGetErrorString(e));
   h->topo_valid = 1;
   return 0;
 }
@@ -1203,6 +1250,8 @@ int agimp_batch_destroy(agimp_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   b->graph[0].release();
   b->graph[1].release();
+  b->chain[0].release();
+  b->chain[1].release();
   if (b->dd) (void)hipFree(b->dd);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1237,6 +1286,8 @@ int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps) {
     b->dims = m;
     b->graph[0].release();
     b->graph[1].release();
+    b->chain[0].release();
+    b->chain[1].release();
   }
   // per-scene dt in the descriptors (read by the kernels at run time)
   for (int i = 0; i < n; ++i) {
@@ -1249,18 +1300,12 @@ int agimp_batch_advance(agimp_batch* b, const float* dt, int32_t steps) {
     }
   }
   if (batch_sync_buffers(b)) return AGIMP_EHIP;
-  for (int p = 0; p < 2; ++p) {
-    if (b->graph[p].exec) continue;
-    hipError_t e = capture_substep(b->stream, b->dd, b->dims, p, b->graph[p]);
-    if (e != hipSuccess) {
-      b->graph[p].release();
-      b->err = std::string("graph capture: ") + hipGetErrorString(e);
-      return AGIMP_EHIP;
-    }
-  }
-  for (int s = 0; s < steps; ++s) {
-    if (hipGraphLaunch(b->graph[b->par].exec, b->stream) != hipSuccess) return AGIMP_EHIP;
-    b->par ^= 1;
+  // all scenes advance together: chains of SUBSTEP_CHAIN substeps per host launch, then single
+  // substeps
+  hipError_t e = replay_substeps(b->stream, b->dd, b->dims, b->graph, b->chain, &b->par, steps);
+  if (e != hipSuccess) {
+    b->err = std::string("graph replay: ") + hipGetErrorString(e);
+    return AGIMP_EHIP;
   }
   if (hipStreamSynchronize(b->stream) != hipSuccess) return AGIMP_EHIP;
   for (agimp_handle* h : b->scenes) {

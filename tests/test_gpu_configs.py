@@ -259,3 +259,57 @@ def test_batched_scenes_against_oracle(pkg, O):
     b.close()
     for s in sims:
         s.close()
+
+
+def test_autotune_batch_picks_fastest_recorded_chunk(pkg):
+    """tune_batch on eight of the batched scenes: for every size class the chosen particles per
+    chunk is one of 64..512 and the fastest of that class's recorded trials, every member runs
+    with its class's choice, and tuning leaves the particles as they were (each trial restores
+    the snapshot taken before tuning)."""
+    from importlib import import_module
+
+    at = import_module(pkg.__name__ + ".autotune")
+    wls = pkg.scenes.c4_batch(n_scenes=8, rho=1000.0)
+    sims = [pkg.AdaptiveGIMP(2, w.level, w.coarsest_size, w.n, w.initializer(), **w.params) for w in wls]
+    b = pkg.Batch(sims)
+    before = [s.x_p.to_numpy() for s in sims]
+    ch, log = at.tune_batch(b, [w.dt for w in wls], steps=2)
+    classes = at.size_classes([w.n for w in wls])
+    assert len(ch) == len(wls)
+    for c in set(int(k) for k in classes):
+        trials = [t for t in log if t["class"] == c]
+        assert sorted(t["chunk"] for t in trials) == [64, 128, 256, 512]
+        assert all(t["s_per_substep"] > 0 for t in trials)
+        best = min(trials, key=lambda t: t["s_per_substep"])["chunk"]
+        assert all(int(ch[i]) == best for i in range(len(wls)) if classes[i] == c), (c, trials, ch)
+        assert all(int(s._cfg.particles_per_chunk) > 0 for s in sims)
+    for s, x0 in zip(sims, before):
+        assert np.array_equal(s.x_p.to_numpy(), x0)
+    b.advance([w.dt for w in wls], 2)  # the tuned batch runs
+    assert all(np.isfinite(s.x_p.to_numpy()).all() for s in sims)
+    b.close()
+    for s in sims:
+        s.close()
+
+
+@pytest.mark.parametrize("n", [8, 11, 19])
+def test_chained_replays_equal_single_replays(pkg, O, n):
+    """advance() replays chains of 8 substeps per graph launch, then single substeps: the result
+    equals n single eager substeps up to float32 summation order, and octrees rebuilt from both
+    snapshots agree bit for bit."""
+    wl = pkg.scenes.small_3d(seed=3, n=6000, L=3, c=16)
+    a = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, wl.n, wl.initializer(), **wl.params)
+    e = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, wl.n, wl.initializer(), **wl.params)
+    a.advance(wl.dt, n)
+    for _ in range(n):
+        e.substep(wl.dt)
+    xa, xe = a.x_p.to_numpy(), e.x_p.to_numpy()
+    assert np.abs(xa - xe).max() <= 1e-6
+    assert np.abs(a.v_p.to_numpy() - e.v_p.to_numpy()).max() <= 1e-4 * max(np.abs(e.v_p.to_numpy()).max(), 1e-30)
+    assert np.array_equal(a.fl_p.to_numpy(), e.fl_p.to_numpy())
+    a.refine()
+    e.refine()
+    for l in range(wl.level):
+        assert np.array_equal(a.cell_mask[l].to_numpy(), e.cell_mask[l].to_numpy()), l
+    a.close()
+    e.close()
