@@ -104,6 +104,24 @@ struct PBuf {
   int64_t stride;
 };
 
+// Field c of slot k: the field's base (the same for every lane, so it stays in scalar registers)
+// plus the slot's byte offset k * 4 (one 32-bit vector register shared by all fields).  Each
+// access is then a single global load or store with a scalar base, without 64-bit address
+// arithmetic per field in vector registers.
+__device__ __forceinline__ float AGIMP_GLOBAL* pfield(const PBuf& P, int c) {
+  float AGIMP_GLOBAL* b = P.f + (size_t)c * P.stride;
+#ifdef __HIP_DEVICE_COMPILE__
+  asm volatile("" : "+s"(b));  // a scalar base the compiler cannot fold into the lane offsets
+#endif
+  return b;
+}
+__device__ __forceinline__ float pload(const float AGIMP_GLOBAL* base, uint32_t off) {
+  return *(const float AGIMP_GLOBAL*)((const char AGIMP_GLOBAL*)base + off);
+}
+__device__ __forceinline__ void pstore(float AGIMP_GLOBAL* base, uint32_t off, float v) {
+  *(float AGIMP_GLOBAL*)((char AGIMP_GLOBAL*)base + off) = v;
+}
+
 // scratch of the (level, leaf block) counting sort and the P2G chunk list
 struct SortBufs {
   int AGIMP_GLOBAL* gb;       // [n]  global block id of each slot's particle (-1: frozen)

@@ -167,10 +167,18 @@ __device__ __forceinline__ void clear_list_grid(const Topo& T) {
 // chained (the bucketing pass follows in the same graph): also zero the grid of the previous
 // grid-phase lists (k_clear_sgrid's work) and leave their counters, which the bucketing pass
 // then neither re-reads for its clear nor needs; the level map resets them.
+// chained: it also does the bucketing pass's preparation that does not depend on the rebuild
+// (zero the per-block particle counters and the frozen counter, count the substep); the level
+// map saves the octree's grid-phase block flags.
 template <int D>
 __global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc, int chained) {
   AGIMP_SCENE
-  if (chained) clear_list_grid<D>(T);
+  if (chained) {
+    clear_list_grid<D>(T);
+    fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
+    fill_zero((uint32_t*)sd.S.frozen, 1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
+  }
   constexpr int WPB = Pow4<D>::v / 4;  // 32-bit words per leaf block of byte masks
   const int nb = T.total_blocks, fbase = T.lv[T.L - 1].gbase;
   uint8_t AGIMP_GLOBAL* sflag = T.lv[0].sflag;
@@ -257,25 +265,22 @@ __global__ void k_clear_sgrid(const SceneDesc* __restrict__ Dsc) {
   clear_list_grid<D>(T);
 }
 
-// before bucketing, in one launch: zero the grid of the previous grid-phase lists (read
-// here, so the list counters are zeroed later, by the bucketing pass), the block counters
-// and the frozen counter; restore the octree's valid-node block flags; count the substep.
-// Chained to a rebuild (same graph, k_clear_refine chained): the grid is already clear and
-// the flags are saved here instead (the same copy the other way).
+// before a bucketing pass that does not follow a rebuild in the same graph, in one launch:
+// zero the grid of the previous grid-phase lists (read here, so the list counters are zeroed
+// later, by the level map), the block counters and the frozen counter; restore the octree's
+// valid-node block flags; count the substep.  (After a rebuild, k_clear_refine and k_levelmap
+// do this work, chained.)
 template <int D>
-__global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc, int chained) {
+__global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const size_t st = (size_t)gridDim.x * blockDim.x;
   const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (!chained) clear_list_grid<D>(T);
+  clear_list_grid<D>(T);
   fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
   fill_zero((uint32_t*)sd.S.frozen, 1);
   uint32_t* a = (uint32_t*)T.lv[0].sflag;
   uint32_t* v = sd.vflag;
-  if (chained)
-    for (size_t i = t0; i < sd.w_flags; i += st) v[i] = a[i];
-  else
-    for (size_t i = t0; i < sd.w_flags; i += st) a[i] = v[i];
+  for (size_t i = t0; i < sd.w_flags; i += st) a[i] = v[i];
   if (t0 == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
 }
 
@@ -714,8 +719,11 @@ constexpr int LM_TABLE_BITS = 8, LM_TABLE = 1 << LM_TABLE_BITS;  // >= threads p
 constexpr int LM_EMPTY = -2;
 // adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
 // particle's 2^d cells; rank inside its (level, leaf block) bucket (run sums merged per workgroup).
+// save_flags (chained after a rebuild): also save the octree's grid-phase block flags (the
+// chunk pass adds to them next), the part of the bucketing preparation that must follow the
+// rebuild's node marks.
 template <int D>
-__global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
+__global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_flags) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
   const int n = sd.n;
@@ -724,6 +732,12 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
   int* bcnt = sd.S.bcnt;
   int* frozen = sd.S.frozen;
   unsigned long long* cnt = sd.cnt;
+  if (save_flags) {
+    const uint32_t* a = (const uint32_t*)T.lv[0].sflag;
+    uint32_t* v = sd.vflag;
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < sd.w_flags; i += st) v[i] = a[i];
+  }
   // the grid-phase list counters (the prep pass read them; k_scompact appends from zero)
   if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
   if (blockIdx.x * blockDim.x >= (unsigned)n) return;  // whole workgroup past this scene
@@ -1595,16 +1609,17 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
   int pos = blockIdx.x * blockDim.x + threadIdx.x;
   if (pos >= n) return;
   int k = perm[pos];
+  const uint32_t ko = (uint32_t)k * 4u, po = (uint32_t)pos * 4u;  // byte offsets of the slots
   // the position is loaded beside the block id, not after the frozen test; the frozen copy
   // uses it too, so it stays ahead of the branch
   float x[3];
 #pragma unroll
-  for (int a = 0; a < D; ++a) x[a] = Pin.f[(size_t)(PL::X + a) * Pin.stride + k];
+  for (int a = 0; a < D; ++a) x[a] = pload(pfield(Pin, PL::X + a), ko);
   int g = gb[k];
   if (g < 0) {  // frozen (fl_p = -1, adaptive_gimp.py:211-216): carried over unchanged
 #pragma unroll
-    for (int a = 0; a < D; ++a) Pout.f[(size_t)(PL::X + a) * Pout.stride + pos] = x[a];
-    for (int c = PL::X + D; c < PL::NF; ++c) Pout.f[(size_t)c * Pout.stride + pos] = Pin.f[(size_t)c * Pin.stride + k];
+    for (int a = 0; a < D; ++a) pstore(pfield(Pout, PL::X + a), po, x[a]);
+    for (int c = PL::X + D; c < PL::NF; ++c) pstore(pfield(Pout, c), po, pload(pfield(Pin, c), ko));
     Pout.g[pos] = Pin.g[k];
     Pout.id[pos] = Pin.id[k];
     flv[pos] = -1;
@@ -1660,12 +1675,13 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
           }
         }
   }
-  float sc = ph.dt / Ll.dx;  // dt (0.5/r)^d / dx with r = 0.5
+  // dt (0.5/r)^d / dx with r = 0.5 (a power-of-two dx makes the product exact)
+  float sc = T.pow2 ? ph.dt * Ll.inv_dx : ph.dt / Ll.dx;
   float Fo[3][3];
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int j = 0; j < D; ++j) Fo[i][j] = Pin.f[(size_t)(PL::F + i * D + j) * Pin.stride + k];
+    for (int j = 0; j < D; ++j) Fo[i][j] = pload(pfield(Pin, PL::F + i * D + j), ko);
   double Fd[D][D];
 #pragma unroll
   for (int i = 0; i < D; ++i)
@@ -1676,7 +1692,7 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
       for (int q = 0; q < D; ++q) acc += ((i == q ? 1.f : 0.f) + sc * G[i][q]) * Fo[q][j];
       Fd[i][j] = (double)acc;
     }
-  double Jp = (double)Pin.f[(size_t)PL::JP * Pin.stride + k];
+  double Jp = (double)pload(pfield(Pin, PL::JP), ko);
   // everything that does not depend on the SVD is stored first: shorter live ranges across
   // the FP64 SVD, and the stores drain while it runs
   bool finite = true;
@@ -1684,14 +1700,14 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
   for (int a = 0; a < D; ++a) {
     float xn = x[a] + ph.dt * nv[a];
     finite = finite && isfinite(xn) && isfinite(nv[a]);
-    Pout.f[(size_t)(PL::X + a) * Pout.stride + pos] = xn;
-    Pout.f[(size_t)(PL::V + a) * Pout.stride + pos] = nv[a];
+    pstore(pfield(Pout, PL::X + a), po, xn);
+    pstore(pfield(Pout, PL::V + a), po, nv[a]);
   }
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int j = 0; j < D; ++j) Pout.f[(size_t)(PL::C + i * D + j) * Pout.stride + pos] = G[i][j];
-  Pout.f[(size_t)PL::M * Pout.stride + pos] = Pin.f[(size_t)PL::M * Pin.stride + k];
+    for (int j = 0; j < D; ++j) pstore(pfield(Pout, PL::C + i * D + j), po, G[i][j]);
+  pstore(pfield(Pout, PL::M), po, pload(pfield(Pin, PL::M), ko));
   Pout.g[pos] = Pin.g[k];
   Pout.id[pos] = Pin.id[k];
   flv[pos] = l;
@@ -1701,12 +1717,12 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int j = 0; j < D; ++j) Pout.f[(size_t)(PL::F + i * D + j) * Pout.stride + pos] = (float)Fd[i][j];
+    for (int j = 0; j < D; ++j) pstore(pfield(Pout, PL::F + i * D + j), po, (float)Fd[i][j]);
 #pragma unroll
   for (int i = 0; i < D; ++i)
 #pragma unroll
-    for (int j = i; j < D; ++j) Pout.f[(size_t)(PL::TAU + tau_idx<D>(i, j)) * Pout.stride + pos] = (float)tau[i][j];
-  Pout.f[(size_t)PL::JP * Pout.stride + pos] = (float)Jp;
+    for (int j = i; j < D; ++j) pstore(pfield(Pout, PL::TAU + tau_idx<D>(i, j)), po, (float)tau[i][j]);
+  pstore(pfield(Pout, PL::JP), po, (float)Jp);
 }
 
 // Kirchhoff stress of the stored F (no plastic projection): after the caller sets state
@@ -1885,8 +1901,8 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bo
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
   const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {
-    AGIMP_LAUNCH("k_bucket_prep", k_bucket_prep<DD>, fg, dim3(256), 0, s, d, chained ? 1 : 0);
-    AGIMP_LAUNCH("k_levelmap", k_levelmap<DD>, pg, dim3(256), 0, s, d, par);
+    if (!chained) AGIMP_LAUNCH("k_bucket_prep", k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
+    AGIMP_LAUNCH("k_levelmap", k_levelmap<DD>, pg, dim3(256), 0, s, d, par, chained ? 1 : 0);
     AGIMP_LAUNCH("k_scan_reduce", k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     AGIMP_LAUNCH("k_scan_final", k_scan_final, tg, dim3(SCAN_T), 0, s, d);
     AGIMP_LAUNCH("k_scatter_blocks", k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);

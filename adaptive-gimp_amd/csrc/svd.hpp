@@ -34,14 +34,21 @@ AGIMP_HD inline double rsqrt_d(double x) {
 
 AGIMP_HD inline void jacobi_rotate(double A[3][3], double V[3][3], int p, int q) {
   double apq = A[p][q];
-  if (fabs(apq) < 1e-300) return;
-  // t = tan of the rotation angle, the root of t^2 + 2 theta t - 1 = 0 with |t| <= 1,
-  // theta = (aqq - app) / (2 apq); with d = aqq - app and e = 2 apq the same root is
-  // t = sgn(d) e / (|d| + sqrt(d^2 + e^2)), one division instead of three
+  // (below 1e-150, d^2 + e^2 could underflow; such an entry is already zero to FP64 precision
+  // next to any diagonal entry the sweep criterion can still resolve)
+  if (fabs(apq) < 1e-150) return;
+  // The rotation angle phi has tan(phi) = t, the root of t^2 + 2 theta t - 1 = 0 with
+  // |t| <= 1, theta = (aqq - app) / (2 apq).  With d = aqq - app, e = 2 apq and
+  // r = sqrt(d^2 + e^2): cos(2 phi) = |d| / r and sin(2 phi) = sgn(d) e / r, so by the
+  // half-angle formulas cos^2(phi) = (1 + |d|/r) / 2 and sin(phi) = sin(2 phi) / (2 cos(phi)).
+  // Two reciprocal square roots, no division and no square root.
   const double d = A[q][q] - A[p][p], e = 2.0 * apq;
-  double t = (d >= 0.0 ? e : -e) / (fabs(d) + sqrt(d * d + e * e));
-  double c = rsqrt_d(t * t + 1.0);
-  double s = t * c;
+  const double inv_r = rsqrt_d(d * d + e * e);
+  const double c2 = 0.5 + 0.5 * fabs(d) * inv_r;  // in [1/2, 1]
+  const double ic = rsqrt_d(c2);
+  double c = c2 * ic;
+  double s = 0.5 * (d >= 0.0 ? e : -e) * inv_r * ic;
+  double t = s * ic;
   // A' = J^T A J with J the rotation in the (p,q) plane
   int r = 3 - p - q;
   double app = A[p][p], aqq = A[q][q], arp = A[r][p], arq = A[r][q];
@@ -173,7 +180,7 @@ AGIMP_HD inline void constitutive(double F[D][D], double& Jp, const Physics& P, 
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       double ns = fmin(fmax(s[a], lo), hi);
-      Jp *= s[a] / ns;
+      if (ns != s[a]) Jp *= s[a] / ns;  // s / ns is exactly 1 otherwise
       s[a] = ns;
     }
 #pragma unroll
