@@ -1077,6 +1077,41 @@ __device__ __forceinline__ float seg_sum(float v, const SegMask& m) {
   return v;
 }
 
+// The wavefront-wide scan of four quantities at once, each step one fused multiply-add with a
+// DPP source: v += v[source lane] * on, with on = 1 where the step adds and 0 where it does
+// not (bound_ctrl makes a missing source lane read 0).  The same sums, in the same order, as
+// seg_sum<true> on each quantity (x + y * 1 is x + y in one rounding; x + y * 0 is x), in
+// 24 instructions instead of 48 (the compiler does not fold a DPP move into the tied
+// accumulator of v_fmac).  Inside the block each quantity's next step comes 4 instructions
+// after its last write, more than the 2 wait states a DPP read of a VALU result needs; the
+// leading s_nop 1 gives them for the writes before the block.  A non-finite value in one
+// segment can reach its neighbour here (inf * 0), which the selecting scan avoids; such a
+// particle is already counted as non-finite by G2P.
+struct SegOn {
+  float s1, s2, s4, s8, s16, s32;
+};
+__device__ __forceinline__ SegOn seg_on(const SegMask& m) {
+  return SegOn{m.s1 ? 1.f : 0.f, m.s2 ? 1.f : 0.f, m.s4 ? 1.f : 0.f,
+               m.s8 ? 1.f : 0.f, m.s16 ? 1.f : 0.f, m.s32 ? 1.f : 0.f};
+}
+#define AGIMP_SEG_STEP(ctrl, on)                                                      \
+  "v_fmac_f32_dpp %0, %0, " on " " ctrl " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+  "v_fmac_f32_dpp %1, %1, " on " " ctrl " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+  "v_fmac_f32_dpp %2, %2, " on " " ctrl " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t" \
+  "v_fmac_f32_dpp %3, %3, " on " " ctrl " row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+__device__ __forceinline__ void seg_sum4_wave(float& a, float& b, float& c, float& d, const SegOn& o) {
+  asm volatile("s_nop 1\n\t"
+               AGIMP_SEG_STEP("row_shr:1", "%4")
+               AGIMP_SEG_STEP("row_shr:2", "%5")
+               AGIMP_SEG_STEP("row_shr:4", "%6")
+               AGIMP_SEG_STEP("row_shr:8", "%7")
+               AGIMP_SEG_STEP("row_bcast:15", "%8")
+               AGIMP_SEG_STEP("row_bcast:31", "%9")
+               : "+v"(a), "+v"(b), "+v"(c), "+v"(d)
+               : "v"(o.s1), "v"(o.s2), "v"(o.s4), "v"(o.s8), "v"(o.s16), "v"(o.s32));
+}
+#undef AGIMP_SEG_STEP
+
 // ------------------------------------------------------------------------------ transfers
 // Quadratic B-spline weights and their t-derivatives: for r = 0.5 the GIMP overlap
 // integrals of utils.py:18-42 summed by adaptive_gimp.py:218-228 reduce to these.
@@ -1262,6 +1297,7 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
         // (dw0 w1 w2, w0 dw1 w2, w0 w1 dw2), regrouped so that most of it is computed once per
         // (i0, i1):  p_a = w2 F_a + w01 E_a(i2) + mw i2 C_a2, where
         // F_a = w01 m (vb_a + C_a0 i0 + C_a1 i1) + S_a0 dw0 w1 + S_a1 w0 dw1, E_a(i2) = S_a2 dw2
+        const SegOn on = seg_on(sm);
         float E[3][3], C2[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -1292,10 +1328,7 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
                 if (i2 == 2) pa += mw * C2[a];
                 p[a] = pa;
               }
-              p[0] = seg_sum<WAVE>(p[0], sm);
-              p[1] = seg_sum<WAVE>(p[1], sm);
-              p[2] = seg_sum<WAVE>(p[2], sm);
-              mw = seg_sum<WAVE>(mw, sm);
+              seg_sum4_wave(p[0], p[1], p[2], mw, on);
               if (writer) {
                 float4& t = tile[tid >> 6][((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)];
                 float4 u = t;

@@ -559,3 +559,53 @@ def test_refine_after_moving_everything_leaves_no_stale_marks(pkg, O, dim):
     ora.refine()
     assert_topology_equal(sim, ora, O, L)
     sim.close()
+
+
+def test_p2g_one_particle_per_block(pkg, O):
+    """One particle per leaf block: 32768 P2G chunks of one particle each, whose node tiles
+    overlap their neighbours' in every direction."""
+    rng = np.random.default_rng(11)
+    k = (4 * np.arange(32) + 1.5) / 128.0
+    x = np.stack(np.meshgrid(k, k, k, indexing="ij"), -1).reshape(-1, 3)
+    n = x.shape[0]
+    x = x + rng.uniform(-0.2, 0.2, x.shape) / 128.0
+    st = dict(x=x, v=rng.standard_normal((n, 3)), F=np.tile(np.eye(3), (n, 1, 1)), C=np.zeros((n, 3, 3)),
+              m=np.ones(n), g=np.zeros(n, np.int32))
+    sim, ora = make_pair(pkg, O, 3, 1, 128, st, {})
+    sim.reset_counters()
+    sim.substep(1e-5)
+    ora.substep(1e-5)
+    assert sim.counters()["chunks"] == n
+    assert_grid_close(sim, ora, 1)
+    assert_particles_close(particle_errors(sim, ora))
+    for _ in range(3):
+        sim.substep(1e-5)
+        ora.substep(1e-5)
+    e = particle_errors(sim, ora)
+    assert e["fl"] == 0 and e["x"] <= 1e-5 and e["v"] <= 1e-3 and e["F"] <= 1e-4, e
+    sim.close()
+
+
+def test_p2g_blocks_split_over_chunks(pkg, O):
+    """64-particle chunks: some occupied leaf blocks of the plastic 3D scene hold several
+    chunks, whose node tiles add into the same grid nodes."""
+    dim, L, c, st, prm, gi, dt = scene(pkg, "small3d_plastic")
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    sim.reset_counters()
+    sim.substep(dt)
+    whole = sim.counters()["chunks"]
+    sim.close()
+    sim, ora = make_pair(pkg, O, dim, L, c, st, prm, gi)
+    sim.set_particles_per_chunk(64)
+    sim.reset_counters()
+    sim.substep(dt)
+    ora.substep(dt)
+    assert sim.counters()["chunks"] > whole  # some blocks now hold several chunks
+    assert_grid_close(sim, ora, L)
+    assert_particles_close(particle_errors(sim, ora))
+    for _ in range(4):
+        sim.substep(dt)
+        ora.substep(dt)
+    e = particle_errors(sim, ora)
+    assert e["fl"] == 0 and e["x"] <= 1e-5 and e["v"] <= 1e-3 and e["F"] <= 1e-4, e
+    sim.close()
