@@ -18,7 +18,12 @@ HIP path's boundary conditions, plasticity and static activation against closed 
 The 2D workloads run with rho = 1000 where several substeps are compared: as written (rho = 1)
 they break the P-wave CFL bound at their time step, so round-off grows exponentially
 (tests/test_oracle_pins.py::test_c1_as_specified_is_unstable shows it for the two squares).
+
+It also checks that PyTorch's GPU runtime, initialised first, shares a process with the
+library.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -313,3 +318,35 @@ def test_chained_replays_equal_single_replays(pkg, O, n):
         assert np.array_equal(a.cell_mask[l].to_numpy(), e.cell_mask[l].to_numpy()), l
     a.close()
     e.close()
+
+
+_TORCH_FIRST = r"""
+import sys
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+import torch
+a = torch.arange(1 << 16, device="cuda", dtype=torch.float32)
+s0 = float((2 * a).sum().item())
+import __graft_entry__ as g
+pkg = g.package()
+wl = pkg.scenes.small_3d(seed=1, n=6000, L=3, c=16)
+sim = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, wl.n, wl.initializer(), None, **wl.params)
+sim.advance(wl.dt, 3)
+x = sim.x_p.to_numpy()
+sim.close()
+s1 = float((2 * a).sum().item())
+assert np.isfinite(x).all() and s0 == s1 == float(2 * sum(range(1 << 16)))
+print("torch-then-library ok")
+"""
+
+
+def test_torch_initialised_before_the_library_coexists():
+    """PyTorch's GPU runtime and the library share one process if torch initialises first (the
+    order bench.py --copy-mb uses); the other order fails in torch's initialisation.  Run in a
+    fresh process, since this one has initialised the library already."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", _TORCH_FIRST, root], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and "torch-then-library ok" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
