@@ -1318,6 +1318,13 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
               F[a] = m01 * (vb[a] + Cm[a][0] * i0 + Cm[a][1] * i1) + (S[a][0] * h0 + S[a][1] * h1);
 #pragma unroll
             for (int i2 = 0; i2 < 3; ++i2) {
+              // the tile read is issued first (after the previous node's write, which it must
+              // follow), so the node's arithmetic and scan hide its latency.  Every lane reads
+              // and adds (its node lies in the tile; only tails write), and the sum is used on
+              // every path (the empty asm below): a read whose wait sits only in the tails'
+              // branch makes the next read wait for it again before reusing its registers.
+              float4& t = tile[tid >> 6][((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)];
+              float4 u = t;
               const float w2 = w[2][i2];
               float mw = m01 * w2;
               float p[3];
@@ -1329,12 +1336,9 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
                 p[a] = pa;
               }
               seg_sum4_wave(p[0], p[1], p[2], mw, on);
-              if (writer) {
-                float4& t = tile[tid >> 6][((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)];
-                float4 u = t;
-                u.x += p[0]; u.y += p[1]; u.z += p[2]; u.w += mw;
-                t = u;
-              }
+              u.x += p[0]; u.y += p[1]; u.z += p[2]; u.w += mw;
+              if (writer) t = u;
+              asm volatile("" ::"v"(u.x), "v"(u.y), "v"(u.z), "v"(u.w));
             }
           }
       }
