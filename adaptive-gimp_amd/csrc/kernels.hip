@@ -759,19 +759,38 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
       }
       const LevelDesc Ll = T.lv[l];
       int b[3];
+      // per axis: the two cells' address terms and whether each lies in the padded box (the
+      // block-major index is a sum of per-axis terms, so the 2^d cells need 2d terms)
+      uint32_t t0[3], t1[3];
+      bool in0[3], in1[3];
 #pragma unroll
-      for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Ll, T.pow2);
+      for (int a = 0; a < D; ++a) {
+        b[a] = base_cell(x[a], Ll, T.pow2);
+        const int j = b[a] + PAD;
+        in0[a] = j >= 0 && j < Ll.E;
+        in1[a] = j + 1 >= 0 && j + 1 < Ll.E;
+        t0[a] = bidx_axis<D>(Ll.NB, a, j);
+        t1[a] = bidx_axis<D>(Ll.NB, a, j + 1);
+      }
       bool hit = false;
 #pragma unroll
       for (int u = 0; u < (1 << D); ++u) {
-        int I[3];
+        bool in = true;
+        uint32_t q = 0;
 #pragma unroll
-        for (int a = 0; a < D; ++a) I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
-        int64_t q = lidx<D>(Ll, I);
-        const uint8_t c = (q >= 0) ? Ll.cell[q] : (uint8_t)0;
+        for (int a = 0; a < D; ++a) {
+          const bool hi = (u >> (D - 1 - a)) & 1;
+          in = in && (hi ? in1[a] : in0[a]);
+          q += hi ? t1[a] : t0[a];
+        }
+        const uint8_t c = in ? Ll.cell[q] : (uint8_t)0;
         hit = hit || c == ACTIVATED;
-        if (l == gp && q >= 0 && !(c & (ACTIVATED | COVERED)))
+        if (l == gp && in && !(c & (ACTIVATED | COVERED))) {
+          int I[3];
+#pragma unroll
+          for (int a = 0; a < D; ++a) I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
           viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
+        }
       }
       if (hit && fl < 0) fl = l;
     }
