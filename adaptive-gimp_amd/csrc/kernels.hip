@@ -72,6 +72,31 @@ __device__ __forceinline__ bool decode_list(const Topo& T, const int* list, int 
   return true;
 }
 
+// decode_list for a thread index t that runs over the list contiguously with lane 0 of every
+// wavefront at a multiple of 64 (workgroups of a multiple of 64 threads): in 3D a leaf block
+// has 64 cells, so all lanes of a wavefront share the list entry, and its load, level and
+// block coordinates are computed once per wavefront in scalar registers
+template <int D>
+__device__ __forceinline__ bool decode_list_wave(const Topo& T, const int* list, int count, int t, int& l, int* I,
+                                                 uint32_t& q) {
+  if constexpr (D == 3) {
+    const int e = __builtin_amdgcn_readfirstlane(t >> 6), w = t & 63;
+    if (e >= count) return false;
+    const int g = __builtin_amdgcn_readfirstlane(list[e]);
+    l = level_of_gbid(T, g);
+    const int NB = T.lv[l].NB, b = g - T.lv[l].gbase;
+    int B[3];
+    block_coords<D>(NB, b, B);
+    I[0] = B[0] * 4 + (w >> 4) - PAD;
+    I[1] = B[1] * 4 + ((w >> 2) & 3) - PAD;
+    I[2] = B[2] * 4 + (w & 3) - PAD;
+    q = (uint32_t)b * Pow4<D>::v + w;
+    return true;
+  } else {
+    return decode_list<D>(T, list, count, t, l, I, q);
+  }
+}
+
 // flag the leaf block holding cell q of level l as non-empty
 template <int D>
 __device__ __forceinline__ void mark_cell_block(const Topo& T, int l, int64_t q) {
@@ -325,16 +350,31 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
 #pragma unroll
   for (int a = 0; a < D; ++a)
     if (((b[a] + PAD) >> 2) != ((b[a] + 1 + PAD) >> 2)) cross |= 1 << (D - 1 - a);
+  // the corners' addresses from per-axis terms (the block-major index is their sum)
+  uint32_t t0[3], t1[3];
+  bool in0[3], in1[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const int j = b[a] + PAD;
+    in0[a] = j >= 0 && j < Lg.E;
+    in1[a] = j + 1 >= 0 && j + 1 < Lg.E;
+    t0[a] = bidx_axis<D>(Lg.NB, a, j);
+    t1[a] = bidx_axis<D>(Lg.NB, a, j + 1);
+  }
 #pragma unroll
   for (int t = 0; t < (1 << D); ++t) {
-    int I[3];
+    bool in = true;
+    uint32_t q = 0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) I[a] = b[a] + ((t >> (D - 1 - a)) & 1);
-    int64_t q = lidx<D>(Lg, I);
-    if (q < 0) { count(cnt, CNT_OOB); continue; }
+    for (int a = 0; a < D; ++a) {
+      const bool hi = (t >> (D - 1 - a)) & 1;
+      in = in && (hi ? in1[a] : in0[a]);
+      q += hi ? t1[a] : t0[a];
+    }
+    if (!in) { count(cnt, CNT_OOB); continue; }
     if (dup) continue;
     Lg.cell[q] = ACTIVATED;  // cells only ever receive ACTIVATED in this phase
-    if ((t & ~cross) == 0) mark_cell_block<D>(T, g, q);
+    if ((t & ~cross) == 0) T.cflag[Lg.gbase + (int)(q / Pow4<D>::v)] = 1;
   }
 }
 
@@ -419,7 +459,7 @@ __global__ void k_mark_hier(const SceneDesc* __restrict__ Dsc) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
     int l, I[3];
     uint32_t q;
-    if (!decode_list<D>(T, list, nlist, t, l, I, q) || l < 1) continue;
+    if (!decode_list_wave<D>(T, list, nlist, t, l, I, q) || l < 1) continue;
     if (!(T.lv[l].cell[q] & ACTIVATED)) continue;
     int l0 = -1;
     for (int lq = 0; lq < l; ++lq) {
@@ -457,7 +497,7 @@ __global__ void k_split(const SceneDesc* __restrict__ Dsc) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
     int l, I[3];
     uint32_t q;
-    if (!decode_list<D>(T, list, nlist, t, l, I, q) || l > T.L - 2) continue;
+    if (!decode_list_wave<D>(T, list, nlist, t, l, I, q) || l > T.L - 2) continue;
     if (!(T.lv[l].cell[q] & CULLING)) continue;
 #pragma unroll
     for (int u = 0; u < (1 << D); ++u) {
@@ -484,7 +524,7 @@ __global__ void k_node_A(const SceneDesc* __restrict__ Dsc) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
   int l, I[3];
   uint32_t q;
-  if (!decode_list<D>(T, list, nlist, t, l, I, q)) continue;
+  if (!decode_list_wave<D>(T, list, nlist, t, l, I, q)) continue;
   if (T.lv[l].cell[q] != ACTIVATED) continue;
 #pragma unroll
   for (int u = 0; u < (1 << D); ++u) {
@@ -542,7 +582,7 @@ __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
   int l, I[3];
   uint32_t q;
   uint32_t word = 0;
-  if (decode_list<D>(T, list, nlist, t, l, I, q) && T.lv[l].cell[q] == ACTIVATED) {
+  if (decode_list_wave<D>(T, list, nlist, t, l, I, q) && T.lv[l].cell[q] == ACTIVATED) {
 #pragma unroll
   for (int j = 0; j < 2 * D; ++j) {
     int I1[3];
