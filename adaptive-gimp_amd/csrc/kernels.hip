@@ -539,33 +539,65 @@ __global__ void k_node_A(const SceneDesc* __restrict__ Dsc) {
   }
 }
 
-// shared face logic of adaptive_gimp.py:160-169; returns l_p or -1 if the face is skipped
+// The face logic of adaptive_gimp.py:155-169 for the 2d faces of cell I at once: lp[j] is
+// the level of the activated cell that holds face j's neighbour I +- e_{j/2} (the hierarchical
+// check, the finest such level), or -1 if the face is skipped (neighbour outside the level,
+// none activated, or the neighbour itself activated at level l).  Per level, the neighbours' cells differ from I's only on one axis, so their
+// addresses are sums of the cell's own per-axis terms and one neighbour term.
 template <int D>
-__device__ __forceinline__ int face_parent(const Topo& T, int l, const int* I, int j, int* I1) {
-  int ax = j >> 1;
+__device__ __forceinline__ void face_parents(const Topo& T, int l, const int* I, int* lp) {
+  const int n = T.lv[l].n;
+  bool own[3], nb_in[3][2];  // I_a in [0, n); I_a + 1 / I_a - 1 in [0, n)
 #pragma unroll
-  for (int a = 0; a < D; ++a) I1[a] = I[a];
-  I1[ax] += (j & 1) ? -1 : 1;
-  bool inr = true;
+  for (int a = 0; a < D; ++a) {
+    own[a] = I[a] >= 0 && I[a] < n;
+    nb_in[a][0] = I[a] + 1 >= 0 && I[a] + 1 < n;
+    nb_in[a][1] = I[a] - 1 >= 0 && I[a] - 1 < n;
+  }
+  bool inr[2 * D];
 #pragma unroll
-  for (int a = 0; a < D; ++a) inr = inr && (I1[a] >= 0) && (I1[a] < T.lv[l].n);
-  if (!inr) return -1;
-  // _heriarchical_check :155-158 (unrolled over the level bound: the cell loads of all
-  // levels are independent and issue together)
-  int lp = -1;
+  for (int j = 0; j < 2 * D; ++j) {
+    bool ok = nb_in[j >> 1][j & 1];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      if (a != (j >> 1)) ok = ok && own[a];
+    inr[j] = ok;
+    lp[j] = -1;
+  }
+  // unrolled over the level bound: the cell loads of all levels are independent and issue
+  // together
 #pragma unroll
   for (int lq = 0; lq < MAXL; ++lq) {
     if (lq <= l) {
-      int P[3];
-#pragma unroll
-      for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - lq);
+      const int sft = l - lq;
       const LevelDesc Lq = T.lv[lq];
-      int64_t qq = lidx<D>(Lq, P);
-      if (qq >= 0 && Lq.cell[qq] == ACTIVATED) lp = lq;
+      uint32_t t0[3], tn[3][2];
+      bool i0[3], in_n[3][2];
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        const int c0 = (I[a] >> sft) + PAD, cp = ((I[a] + 1) >> sft) + PAD, cm = ((I[a] - 1) >> sft) + PAD;
+        i0[a] = c0 >= 0 && c0 < Lq.E;
+        in_n[a][0] = cp >= 0 && cp < Lq.E;
+        in_n[a][1] = cm >= 0 && cm < Lq.E;
+        t0[a] = bidx_axis<D>(Lq.NB, a, c0);
+        tn[a][0] = bidx_axis<D>(Lq.NB, a, cp);
+        tn[a][1] = bidx_axis<D>(Lq.NB, a, cm);
+      }
+#pragma unroll
+      for (int j = 0; j < 2 * D; ++j) {
+        const int ax = j >> 1, sd1 = j & 1;
+        bool in = inr[j] && in_n[ax][sd1];
+        uint32_t qq = tn[ax][sd1];
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+          if (a != ax) { in = in && i0[a]; qq += t0[a]; }
+        if (in && Lq.cell[qq] == ACTIVATED) lp[j] = lq;
+      }
     }
   }
-  if (lp == l) return -1;  // cell_mask[l][I1] == ACTIVATED
-  return lp;
+#pragma unroll
+  for (int j = 0; j < 2 * D; ++j)
+    if (lp[j] == l) lp[j] = -1;  // cell_mask[l][I1] == ACTIVATED
 }
 
 // adaptive_gimp.py:170-174 for all levels at once.  The coarse-fine faces found here are also
@@ -583,36 +615,50 @@ __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
   uint32_t q;
   uint32_t word = 0;
   if (decode_list_wave<D>(T, list, nlist, t, l, I, q) && T.lv[l].cell[q] == ACTIVATED) {
+    int lp[2 * D];
+    face_parents<D>(T, l, I, lp);
+    // the nodes of the face shared with neighbour I +- e_ax are corners of this cell (corner
+    // bit 1 on axis ax for the + side, 0 for the - side); their node-mask addresses on the
+    // finest level from per-axis terms
+    const LevelDesc Lf = T.lv[T.L - 1];
+    const int sh = T.lv[l].shift;
+    uint32_t ct[3][2];
+    bool cin[3][2];
 #pragma unroll
-  for (int j = 0; j < 2 * D; ++j) {
-    int I1[3];
-    int lp = face_parent<D>(T, l, I, j, I1);
-    if (lp < 0) continue;
-    word |= (uint32_t)(lp + 1) << (3 * j);
-    int ax = j >> 1, side = j & 1;
-    int msk = (1 << (l - lp)) - 1;
+    for (int a = 0; a < D; ++a)
 #pragma unroll
-    for (int u = 0; u < (1 << D); ++u) {
-      int kk[3];
-#pragma unroll
-      for (int a = 0; a < D; ++a) kk[a] = (u >> (D - 1 - a)) & 1;
-      if (kk[ax] != side) continue;
-      bool any = false;
-      int f[3];
-#pragma unroll
-      for (int a = 0; a < D; ++a) {
-        int c = I1[a] + kk[a];
-        any = any || ((c & msk) != 0);
-        f[a] = c << T.lv[l].shift;
+      for (int k = 0; k < 2; ++k) {
+        const int j = ((I[a] + k) << sh) + PAD;
+        cin[a][k] = j >= 0 && j < Lf.E;
+        ct[a][k] = bidx_axis<D>(Lf.NB, a, j);
       }
-      if (!any) continue;
-      int64_t qq = lidx<D>(T.lv[T.L - 1], f);
-      if (qq < 0) { count(cnt, CNT_OOB); continue; }
-      // a T node lies on the face this cell shares with I1, so it is a corner of this
-      // activated cell: k_node_A has already flagged its grid-phase blocks
-      T.node[qq] |= T_JUNCTION;
+#pragma unroll
+    for (int j = 0; j < 2 * D; ++j) {
+      if (lp[j] < 0) continue;
+      word |= (uint32_t)(lp[j] + 1) << (3 * j);
+      const int ax = j >> 1, cb = 1 - (j & 1);  // the face's corner bit on axis ax
+      const int msk = (1 << (l - lp[j])) - 1;
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        int kk[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) kk[a] = (u >> (D - 1 - a)) & 1;
+        if (kk[ax] != cb) continue;
+        bool any = false, in = true;
+        uint32_t qq = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          any = any || (((I[a] + kk[a]) & msk) != 0);
+          in = in && cin[a][kk[a]];
+          qq += ct[a][kk[a]];
+        }
+        if (!any) continue;
+        if (!in) { count(cnt, CNT_OOB); continue; }
+        // a T node lies on the face this cell shares with its neighbour, so it is a corner
+        // of this activated cell: k_node_A has already flagged its grid-phase blocks
+        T.node[qq] |= T_JUNCTION;
+      }
     }
-  }
   }
   T.fword[t] = word;
   }
