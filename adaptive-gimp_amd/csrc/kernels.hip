@@ -126,30 +126,75 @@ __device__ __forceinline__ void count(unsigned long long* cnt, int which, unsign
   atomicAdd(&cnt[which], v);
 }
 
-// mark the grid-phase blocks of every level on which finest node f is a node
+// The 2^d corners (I + k, k in {0,1}^d) of level-l cell I as finest-level nodes: their
+// node-mask addresses and in-box flags, from per-axis terms (corner u: bit D-1-a of u is k_a)
 template <int D>
-__device__ __forceinline__ void mark_valid_node(const Topo& T, const int* f) {
-  // unrolled over the level bound: the level descriptors are read once per kernel, not per
-  // node and level (from the finest level down; a node stops being one at the first level
-  // whose spacing does not divide it, and stays so at all coarser ones)
-  bool alive = true;
+struct Corners {
+  uint32_t t[3][2];
+  bool in[3][2];
+  __device__ __forceinline__ Corners(const Topo& T, int l, const int* I) {
+    const LevelDesc Lf = T.lv[T.L - 1];
+    const int sh = T.lv[l].shift;
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int j = ((I[a] + k) << sh) + PAD;
+        in[a][k] = j >= 0 && j < Lf.E;
+        t[a][k] = bidx_axis<D>(Lf.NB, a, j);
+      }
+  }
+  __device__ __forceinline__ bool addr(int u, uint32_t& q) const {
+    bool ok = true;
+    q = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const int k = (u >> (D - 1 - a)) & 1;
+      ok = ok && in[a][k];
+      q += t[a][k];
+    }
+    return ok;
+  }
+};
+
+// Flag the grid-phase blocks of every level on which the corners of level-l cell I selected
+// by the bits of `sel` are nodes: per level, a corner is a node there if its finest
+// coordinates are multiples of that level's spacing, and its block flag index is a sum of
+// per-axis terms.  Unrolled over the level bound, so the level descriptors are read once.
+template <int D>
+__device__ __forceinline__ void mark_valid_corners(const Topo& T, int l, const int* I, uint32_t sel) {
+  if (!sel) return;
+  const int sh = T.lv[l].shift;
 #pragma unroll
   for (int i = 0; i < MAXL; ++i) {
-    const int l = T.L - 1 - i;
-    if (l >= 0) {
-      const LevelDesc Lv = T.lv[l];
+    const int lv = T.L - 1 - i;
+    if (lv >= 0) {
+      const LevelDesc Lv = T.lv[lv];
       const int msk = (1 << Lv.shift) - 1;
-      bool div = true;
+      bool ok[3][2];
+      int bt[3][2];
 #pragma unroll
-      for (int a = 0; a < D; ++a) div = div && ((f[a] & msk) == 0);
-      alive = alive && div;
-      int J[3];
+      for (int a = 0; a < D; ++a)
 #pragma unroll
-      for (int a = 0; a < D; ++a) J[a] = (f[a] >> Lv.shift) + PAD;
-      if (alive && in_box<D>(Lv.E, J)) {
-        const int b = (D == 2) ? ((J[0] >> 2) * Lv.NB + (J[1] >> 2))
-                               : (((J[0] >> 2) * Lv.NB + (J[1] >> 2)) * Lv.NB + (J[2] >> 2));
-        Lv.sflag[b] = 1;
+        for (int k = 0; k < 2; ++k) {
+          const int f = (I[a] + k) << sh;
+          const int J = (f >> Lv.shift) + PAD;
+          ok[a][k] = (f & msk) == 0 && J >= 0 && J < Lv.E;
+          const int stride = (D == 2) ? (a == 0 ? Lv.NB : 1) : (a == 0 ? Lv.NB * Lv.NB : (a == 1 ? Lv.NB : 1));
+          bt[a][k] = (J >> 2) * stride;
+        }
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        if (!((sel >> u) & 1u)) continue;
+        bool alive = true;
+        int b = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          const int k = (u >> (D - 1 - a)) & 1;
+          alive = alive && ok[a][k];
+          b += bt[a][k];
+        }
+        if (alive) Lv.sflag[b] = 1;
       }
     }
   }
@@ -526,16 +571,16 @@ __global__ void k_node_A(const SceneDesc* __restrict__ Dsc) {
   uint32_t q;
   if (!decode_list_wave<D>(T, list, nlist, t, l, I, q)) continue;
   if (T.lv[l].cell[q] != ACTIVATED) continue;
+  const Corners<D> C(T, l, I);
+  uint32_t sel = 0;
 #pragma unroll
   for (int u = 0; u < (1 << D); ++u) {
-    int Jn[3], f[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) { Jn[a] = I[a] + ((u >> (D - 1 - a)) & 1); f[a] = Jn[a] << T.lv[l].shift; }
-    int64_t qq = lidx<D>(T.lv[T.L - 1], f);
-    if (qq < 0) { count(cnt, CNT_OOB); continue; }
+    uint32_t qq;
+    if (!C.addr(u, qq)) { count(cnt, CNT_OOB); continue; }
     T.node[qq] = ACTIVATED;  // node_mask only receives ACTIVATED in this phase
-    mark_valid_node<D>(T, f);
+    sel |= 1u << u;
   }
+  mark_valid_corners<D>(T, l, I, sel);
   }
 }
 
@@ -675,18 +720,18 @@ __device__ __forceinline__ void mark_G_face(const Topo& T, unsigned long long* c
     if (qc < 0) { count(cnt, CNT_OOB); continue; }
     T.lv[l0].cell[qc] |= GHOST;
     mark_cell_block<D>(T, l0, qc);
+    const Corners<D> C(T, l0, P);
+    uint32_t sel = 0;
 #pragma unroll
     for (int u = 0; u < (1 << D); ++u) {
-      int f[3];
-#pragma unroll
-      for (int a = 0; a < D; ++a) f[a] = (P[a] + ((u >> (D - 1 - a)) & 1)) << T.lv[l0].shift;
-      int64_t qq = lidx<D>(T.lv[T.L - 1], f);
-      if (qq < 0) { count(cnt, CNT_OOB); continue; }
+      uint32_t qq;
+      if (!C.addr(u, qq)) { count(cnt, CNT_OOB); continue; }
       if (T.node[qq] == 0) {
         T.node[qq] = GHOST;
-        mark_valid_node<D>(T, f);
+        sel |= 1u << u;
       }
     }
+    mark_valid_corners<D>(T, l0, P, sel);
   }
 }
 
