@@ -1630,32 +1630,60 @@ __global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ 
     block_coords<D>(Lc.NB, b, B);
 #pragma unroll
     for (int a = 0; a < D; ++a) f0[a] = 2 * (4 * B[a] - PAD) - 1;  // first fine node (unpadded)
-    for (int i = lane; i < RN; i += 64) {
-      int If[3], r = i, nodd = 0;
+    // the lane's coarse node of this block (lanes < 4^d): its flags load beside the fine
+    // nodes' flags, and its grid value beside their values
+    int I[3], w[3];
+    if (D == 2) { w[0] = (lane >> 2) & 3; w[1] = lane & 3; }
+    else { w[0] = lane >> 4; w[1] = (lane >> 2) & 3; w[2] = lane & 3; }
 #pragma unroll
-      for (int a = D - 1; a >= 0; --a) { If[a] = f0[a] + r % R; r /= R; nodd += If[a] & 1; }
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (in_range_nodes<D>(If, Lf.n)) {
-        const int64_t qf = lidx<D>(Lf, If);
-        if (qf >= 0) {
-          const uint8_t M = nflag<D>(N, Lf.shift, If);
-          if ((nodd == 0 && f_act(M)) || f_T(M) || f_ghost(M)) {
-            const float4 g = Lf.grid[qf];
-            const float wgt = (nodd == 0) ? 1.f : (nodd == 1 ? 0.5f : (nodd == 2 ? 0.25f : 0.125f));
-            v = make_float4(g.x * wgt, g.y * wgt, g.z * wgt, g.w * wgt);
-          }
+    for (int a = 0; a < D; ++a) I[a] = 4 * B[a] - PAD + w[a];
+    const bool mine = lane < Pow4<D>::v;
+    const uint8_t Mc = (mine && in_range_nodes<D>(I, Lc.n)) ? nflag<D>(N, Lc.shift, I) : (uint8_t)0;
+    // the lane's fine nodes i = lane + 64 k, staged in rounds: every node flag load is issued
+    // first, then every value load the flags ask for (a loop with a flag and a value load per
+    // node waits for both, node after node)
+    constexpr int NPL = (RN + 63) / 64;
+    uint32_t qf[NPL];
+    uint8_t M[NPL];
+    int nodd[NPL];
+    bool use[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+      const int i = lane + 64 * k;
+      int If[3], r = i;
+      nodd[k] = 0;
+#pragma unroll
+      for (int a = D - 1; a >= 0; --a) { If[a] = f0[a] + r % R; r /= R; nodd[k] += If[a] & 1; }
+      use[k] = false;
+      qf[k] = 0;
+      M[k] = 0;
+      if (i < RN && in_range_nodes<D>(If, Lf.n)) {
+        const int64_t q = lidx<D>(Lf, If);
+        if (q >= 0) {
+          qf[k] = (uint32_t)q;
+          M[k] = nflag<D>(N, Lf.shift, If);
+          use[k] = true;
         }
       }
-      tile[wv][i] = v;
+    }
+    float4 g[NPL];
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+      use[k] = use[k] && ((nodd[k] == 0 && f_act(M[k])) || f_T(M[k]) || f_ghost(M[k]));
+      g[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (use[k]) g[k] = Lf.grid[qf[k]];
+    }
+    const uint32_t qc = (uint32_t)b * Pow4<D>::v + lane;
+    float4 gc = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (f_valid(Mc)) gc = Lc.grid[qc];
+#pragma unroll
+    for (int k = 0; k < NPL; ++k) {
+      const int i = lane + 64 * k;
+      const float wgt = (nodd[k] == 0) ? 1.f : (nodd[k] == 1 ? 0.5f : (nodd[k] == 2 ? 0.25f : 0.125f));
+      if (i < RN) tile[wv][i] = make_float4(g[k].x * wgt, g[k].y * wgt, g[k].z * wgt, g[k].w * wgt);
     }
     wave_sync_lds();
-    if (lane < Pow4<D>::v) {
-      int I[3], w[3];
-      if (D == 2) { w[0] = lane >> 2; w[1] = lane & 3; }
-      else { w[0] = lane >> 4; w[1] = (lane >> 2) & 3; w[2] = lane & 3; }
-#pragma unroll
-      for (int a = 0; a < D; ++a) I[a] = 4 * B[a] - PAD + w[a];
-      const uint8_t Mc = in_range_nodes<D>(I, Lc.n) ? nflag<D>(N, Lc.shift, I) : (uint8_t)0;
+    {
       if (f_valid(Mc)) {
         constexpr int NS = (D == 2) ? 9 : 27;
         float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1674,11 +1702,10 @@ __global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ 
         // level 0 is complete after this restriction, so its grid update is applied here
         const bool add = acc.x != 0.f || acc.y != 0.f || acc.z != 0.f || acc.w != 0.f;
         if (add || l == 1) {
-          const uint32_t q = (uint32_t)b * Pow4<D>::v + lane;
-          float4 g = Lc.grid[q];
+          float4 g = gc;
           g.x += acc.x; g.y += acc.y; g.z += acc.z; g.w += acc.w;
           const bool upd = (l == 1) && grid_update<D>(T, Lc, ph, I, Mc, g);
-          if (add || upd) Lc.grid[q] = g;
+          if (add || upd) Lc.grid[qc] = g;
         }
       }
     }
@@ -1722,13 +1749,14 @@ __global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l, float dt_arg
     int J[3];
     uint32_t q = list_node<D>(Lf, t / Pow4<D>::v, t % Pow4<D>::v, J);
     if (!in_range_nodes<D>(J, Lf.n)) continue;
+    // the node's grid value loads beside its flags (every path below needs it)
     uint8_t M = nflag<D>(N, Lf.shift, J);
+    float4 g = Lf.grid[q];
     bool isT = f_T(M), isG = f_ghost(M), isA = f_act(M);
     bool even = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) even = even && ((J[a] & 1) == 0);
     if (!(isT || isG || even)) {  // not overwritten by prolongation: level l's grid update
-      float4 g = Lf.grid[q];
       if (grid_update<D>(T, Lf, ph, J, M, g)) Lf.grid[q] = g;
       continue;
     }
@@ -1771,7 +1799,6 @@ __global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l, float dt_arg
         }
       }
     }
-    float4 g = Lf.grid[q];
     g.x = acc[0]; g.y = acc[1];
     if (D == 3) g.z = acc[2];
     Lf.grid[q] = g;
