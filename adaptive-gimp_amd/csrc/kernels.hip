@@ -881,31 +881,38 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
     for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
     const int gp = sd.check ? P.g[k] : -1;  // level whose request cells are checked
     bool viol = false;
-    // from the finest level down: the first level with an activated cell is fl; below it
-    // only the checked level gp (if any) still needs its cells
-    for (int l = T.L - 1; l >= 0; --l) {
-      if (fl >= 0 && l != gp) {
-        if (gp >= 0 && l > gp) continue;
-        break;
-      }
+    // fl is the finest level with an activated cell among the 2^d; the checked level gp (if
+    // any) needs its cells too.  The cells of all levels (unrolled over the level bound) are
+    // loaded first, all together, then examined.
+    constexpr int NC = 1 << D;
+    uint8_t c[MAXL][NC];
+    uint32_t inb[MAXL];  // bit u: cell u lies in the padded box
+    // the position is consumed here, on the one path into the level loop: otherwise the
+    // level loop's branches leave its load pending at their joins, and the wait for it there
+    // (the memory counter is in order) would also wait for the previous level's cells
+    asm volatile("" ::"v"(x[0]), "v"(x[1]), "v"(x[2]));
+#pragma unroll
+    for (int i = 0; i < MAXL; ++i) {
+      const int l = T.L - 1 - i;
+      inb[i] = 0;
+#pragma unroll
+      for (int u = 0; u < NC; ++u) c[i][u] = 0;
+      if (l < 0) continue;
       const LevelDesc Ll = T.lv[l];
-      int b[3];
       // per axis: the two cells' address terms and whether each lies in the padded box (the
       // block-major index is a sum of per-axis terms, so the 2^d cells need 2d terms)
       uint32_t t0[3], t1[3];
       bool in0[3], in1[3];
 #pragma unroll
       for (int a = 0; a < D; ++a) {
-        b[a] = base_cell(x[a], Ll, T.pow2);
-        const int j = b[a] + PAD;
+        const int j = base_cell(x[a], Ll, T.pow2) + PAD;
         in0[a] = j >= 0 && j < Ll.E;
         in1[a] = j + 1 >= 0 && j + 1 < Ll.E;
         t0[a] = bidx_axis<D>(Ll.NB, a, j);
         t1[a] = bidx_axis<D>(Ll.NB, a, j + 1);
       }
-      bool hit = false;
 #pragma unroll
-      for (int u = 0; u < (1 << D); ++u) {
+      for (int u = 0; u < NC; ++u) {
         bool in = true;
         uint32_t q = 0;
 #pragma unroll
@@ -914,16 +921,30 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
           in = in && (hi ? in1[a] : in0[a]);
           q += hi ? t1[a] : t0[a];
         }
-        const uint8_t c = in ? Ll.cell[q] : (uint8_t)0;
-        hit = hit || c == ACTIVATED;
-        if (l == gp && in && !(c & (ACTIVATED | COVERED))) {
-          int I[3];
-#pragma unroll
-          for (int a = 0; a < D; ++a) I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
-          viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
-        }
+        if (in) { c[i][u] = Ll.cell[q]; inb[i] |= 1u << u; }
       }
+    }
+#pragma unroll
+    for (int i = 0; i < MAXL; ++i) {
+      const int l = T.L - 1 - i;
+      if (l < 0) continue;
+      bool hit = false;
+#pragma unroll
+      for (int u = 0; u < NC; ++u) hit = hit || c[i][u] == ACTIVATED;
       if (hit && fl < 0) fl = l;
+      if (l == gp) {
+        int b[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], T.lv[l], T.pow2);
+#pragma unroll
+        for (int u = 0; u < NC; ++u)
+          if (((inb[i] >> u) & 1u) && !(c[i][u] & (ACTIVATED | COVERED))) {
+            int I[3];
+#pragma unroll
+            for (int a = 0; a < D; ++a) I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
+            viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
+          }
+      }
     }
     if (viol) {
       count(cnt, CNT_VIOL);
