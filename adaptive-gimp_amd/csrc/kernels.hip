@@ -1041,10 +1041,28 @@ __device__ __forceinline__ int2 block_excl_scan(int2 v, int2* sh, int2& total) {
   return make_int2(off.x + incl.x - v.x, off.y + incl.y - v.y);
 }
 
-__device__ __forceinline__ int2 load_cc(const int* bcnt, int i, int n, int CH) {
-  if (i >= n) return make_int2(0, 0);
-  int c = bcnt[i];
-  return make_int2(c, (c + CH - 1) / CH);
+// the (count, chunks) pairs of blocks base .. base + SCAN_I - 1 (0 past n), loaded without a
+// branch per element so the loads issue together: 16-byte loads when all lie below n (base is
+// a multiple of SCAN_I), else clamped element loads
+__device__ __forceinline__ void load_cc(const int* bcnt, int base, int n, int CH, int2* v) {
+  static_assert(SCAN_I % 4 == 0, "whole 16-byte loads");
+  int c[SCAN_I];
+  if (base + SCAN_I <= n) {
+    const int4* p = (const int4*)(bcnt + base);
+#pragma unroll
+    for (int k = 0; k < SCAN_I / 4; ++k) {
+      const int4 a = p[k];
+      c[4 * k] = a.x; c[4 * k + 1] = a.y; c[4 * k + 2] = a.z; c[4 * k + 3] = a.w;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < SCAN_I; ++i) {
+      const int x = bcnt[min(base + i, n - 1)];
+      c[i] = base + i < n ? x : 0;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SCAN_I; ++i) v[i] = make_int2(c[i], (c[i] + CH - 1) / CH);
 }
 
 __global__ void k_scan_reduce(const SceneDesc* __restrict__ Dsc) {
@@ -1055,9 +1073,11 @@ __global__ void k_scan_reduce(const SceneDesc* __restrict__ Dsc) {
   if ((int)blockIdx.x * SCAN_TILE >= n) return;
   __shared__ int2 sh[SCAN_T];
   int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
+  int2 v[SCAN_I];
+  load_cc(bcnt, base, n, CH, v);
   int2 s = make_int2(0, 0);
 #pragma unroll
-  for (int i = 0; i < SCAN_I; ++i) s = add2(s, load_cc(bcnt, base + i, n, CH));
+  for (int i = 0; i < SCAN_I; ++i) s = add2(s, v[i]);
   int2 tot;
   block_excl_scan(s, sh, tot);
   if (threadIdx.x == 0) tile_sum[blockIdx.x] = tot;
@@ -1076,15 +1096,28 @@ __global__ void k_scan_final(const SceneDesc* __restrict__ Dsc) {
   const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
   if ((int)blockIdx.x >= ntiles) return;
   __shared__ int2 sh[SCAN_T];
+  // the preceding tiles' totals, 4 loads in flight per thread and round
   int2 o = make_int2(0, 0);
-  for (int i = threadIdx.x; i < (int)blockIdx.x; i += SCAN_T) o = add2(o, tile_sum[i]);
+  const int nprev = (int)blockIdx.x;
+  for (int i0 = threadIdx.x; i0 < nprev; i0 += 4 * SCAN_T) {
+    int2 w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = i0 + u * SCAN_T;
+      const int2 x = tile_sum[min(i, nprev - 1)];
+      w[u] = i < nprev ? x : make_int2(0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) o = add2(o, w[u]);
+  }
   int2 tile_off;
   block_excl_scan(o, sh, tile_off);  // tile_off = the workgroup's total of o
   int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_I;
   int2 v[SCAN_I];
+  load_cc(bcnt, base, n, CH, v);
   int2 s = make_int2(0, 0);
 #pragma unroll
-  for (int i = 0; i < SCAN_I; ++i) { v[i] = load_cc(bcnt, base + i, n, CH); s = add2(s, v[i]); }
+  for (int i = 0; i < SCAN_I; ++i) s = add2(s, v[i]);
   int2 tot;
   int2 run = add2(block_excl_scan(s, sh, tot), tile_off);
 #pragma unroll
