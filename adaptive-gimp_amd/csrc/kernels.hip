@@ -700,8 +700,10 @@ __global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
         if (!any) continue;
         if (!in) { count(cnt, CNT_OOB); continue; }
         // a T node lies on the face this cell shares with its neighbour, so it is a corner
-        // of this activated cell: k_node_A has already flagged its grid-phase blocks
-        T.node[qq] |= T_JUNCTION;
+        // of this exactly-activated cell: k_node_A has already stored ACTIVATED there and
+        // flagged its grid-phase blocks, and ghost marks come later, so the byte holds A or
+        // A|T and |= T is the plain store of A|T (no read)
+        T.node[qq] = ACTIVATED | T_JUNCTION;
       }
     }
   }
