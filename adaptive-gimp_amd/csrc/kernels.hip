@@ -30,6 +30,8 @@ namespace agimp {
 
 template <int D>
 struct Pow4 { static constexpr int v = (D == 2) ? 16 : 64; };
+template <int D>
+struct Pow3 { static constexpr int v = (D == 2) ? 9 : 27; };
 
 // the scene of this workgroup (blockIdx.y); kernels that touch particles also take the
 // index `par` of the current particle buffer (the other one receives G2P's output)
@@ -1597,6 +1599,16 @@ __device__ __forceinline__ uint8_t nflag(const NodeView& N, int shift, const int
   if (!in_box<D>(N.E, J)) return 0;
   return N.node[bidx<D>(N.NB, J)];
 }
+// nflag without a branch: the load always issues (at index 0 when I lies outside the box)
+template <int D>
+__device__ __forceinline__ uint8_t nflag_nb(const NodeView& N, int shift, const int* I) {
+  int J[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) J[a] = (I[a] << shift) + PAD;
+  const bool ok = in_box<D>(N.E, J);
+  const uint8_t m = N.node[ok ? bidx<D>(N.NB, J) : 0u];
+  return ok ? m : (uint8_t)0;
+}
 __device__ __forceinline__ bool f_valid(uint8_t M) { return M > 0; }
 __device__ __forceinline__ bool f_act(uint8_t M) { return (M & (ACTIVATED | T_JUNCTION)) == ACTIVATED; }
 __device__ __forceinline__ bool f_T(uint8_t M) { return (M & T_JUNCTION) != 0; }
@@ -1791,8 +1803,14 @@ __global__ void k_gridop(const SceneDesc* __restrict__ Dsc, float dt_arg) {
 
 // adaptive_gimp.py:303-321 as a gather onto each level-l node J that is T, ghost or
 // all-even: J = 2I + d over valid coarse nodes I, weight 2^-|d|_1 (only d = 0 unless T/G).
+// A wavefront takes 64 / 4^d leaf blocks of level l, one node per lane.  The sources of a
+// block's nodes 4S - PAD + [0, 4) per axis are the coarse nodes 2S - PAD/2 + [0, 3): the
+// 3^d of them are staged in LDS, their flags and values loaded together (the values
+// whatever the flags say, a read of the dense grid), beside each lane's own node flags and
+// value, so one round of loads precedes the sums.  Sums are taken in the same order as a
+// direct gather.
 template <int D>
-__global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l, float dt_arg) {
+__global__ void __launch_bounds__(256) k_prolong(const SceneDesc* __restrict__ Dsc, int l, float dt_arg) {
   AGIMP_SCENE
   if (l >= T.L) return;
   Physics ph = sd.ph;
@@ -1800,64 +1818,77 @@ __global__ void k_prolong(const SceneDesc* __restrict__ Dsc, int l, float dt_arg
   const LevelDesc Lf = T.lv[l];
   const LevelDesc Lc = T.lv[l - 1];
   const NodeView N = node_view(T);
-  int total = T.scount[l] * Pow4<D>::v;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
-    int J[3];
-    uint32_t q = list_node<D>(Lf, t / Pow4<D>::v, t % Pow4<D>::v, J);
-    if (!in_range_nodes<D>(J, Lf.n)) continue;
-    // the node's grid value loads beside its flags (every path below needs it)
-    uint8_t M = nflag<D>(N, Lf.shift, J);
+  constexpr int NPB = Pow4<D>::v, BPW = 64 / NPB, NCB = Pow3<D>::v, NW = 4;
+  __shared__ float4 src[NW][BPW][NCB];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nblk = T.scount[l];
+  const int sb = lane / NPB, w = lane % NPB;  // this lane's block in the wavefront, node in it
+  const int cb = lane / NCB, cr = lane % NCB;  // the block and source this lane stages
+  for (int e0 = (blockIdx.x * NW + wv) * BPW; e0 < nblk; e0 += gridDim.x * NW * BPW) {  // uniform per wavefront
+    // every load below is issued without a branch (indices clamped, results selected), so
+    // the list entries load in one round and the flags and values in the next
+    const bool stager = lane < BPW * NCB && e0 + cb < nblk;  // coarse source cr of block e0 + cb
+    const int e = e0 + sb;                                    // node w of block e0 + sb
+    const int bs = Lf.slist[min(e0 + cb, nblk - 1)];
+    const int bn = Lf.slist[min(e, nblk - 1)];
+    int B[3], Ic[3], rr = cr;
+    block_coords<D>(Lf.NB, bs, B);
+#pragma unroll
+    for (int a = D - 1; a >= 0; --a) { Ic[a] = 2 * B[a] - PAD / 2 + rr % 3; rr /= 3; }
+    bool sok = stager && in_range_nodes<D>(Ic, Lc.n);
+    const int64_t qs = lidx<D>(Lc, Ic);
+    sok = sok && qs >= 0;
+    const uint8_t Mcs = nflag_nb<D>(N, Lc.shift, Ic);
+    const float4 sv = Lc.grid[sok ? (uint32_t)qs : 0u];
+    int J[3], Bn[3];
+    block_coords<D>(Lf.NB, bn, Bn);
+    if (D == 2) { J[0] = Bn[0] * 4 + (w >> 2) - PAD; J[1] = Bn[1] * 4 + (w & 3) - PAD; }
+    else { J[0] = Bn[0] * 4 + (w >> 4) - PAD; J[1] = Bn[1] * 4 + ((w >> 2) & 3) - PAD; J[2] = Bn[2] * 4 + (w & 3) - PAD; }
+    const uint32_t q = (uint32_t)bn * NPB + w;
+    const bool live = e < nblk && in_range_nodes<D>(J, Lf.n);
+    const uint8_t M = nflag_nb<D>(N, Lf.shift, J);
     float4 g = Lf.grid[q];
-    bool isT = f_T(M), isG = f_ghost(M), isA = f_act(M);
-    bool even = true;
+    if (stager) src[wv][cb][cr] = (sok && f_valid(Mcs)) ? sv : make_float4(0.f, 0.f, 0.f, 0.f);
+    wave_sync_lds();
+    if (live) {
+      const bool isT = f_T(M), isG = f_ghost(M), isA = f_act(M);
+      bool even = true;
 #pragma unroll
-    for (int a = 0; a < D; ++a) even = even && ((J[a] & 1) == 0);
-    if (!(isT || isG || even)) {  // not overwritten by prolongation: level l's grid update
-      if (grid_update<D>(T, Lf, ph, J, M, g)) Lf.grid[q] = g;
-      continue;
-    }
-    float acc[3] = {0.f, 0.f, 0.f};
-    if (isT || isG || isA) {
-      int nodd = 0;
+      for (int a = 0; a < D; ++a) even = even && ((J[a] & 1) == 0);
+      if (!(isT || isG || even)) {  // not overwritten by prolongation: level l's grid update
+        if (grid_update<D>(T, Lf, ph, J, M, g)) Lf.grid[q] = g;
+      } else {
+        float acc[3] = {0.f, 0.f, 0.f};
+        if (isT || isG || isA) {
+          int nodd = 0;
 #pragma unroll
-      for (int a = 0; a < D; ++a) nodd += (J[a] & 1);
-      const int combos = (isT || isG) ? 1 << nodd : (nodd == 0 ? 1 : 0);
-      const float wgt = (nodd == 0) ? 1.f : (nodd == 1 ? 0.5f : (nodd == 2 ? 0.25f : 0.125f));
-      // all source flags first, then the values (independent loads in flight together)
-      int qc[1 << D];
-      uint8_t Mc[1 << D];
+          for (int a = 0; a < D; ++a) nodd += (J[a] & 1);
+          const int combos = (isT || isG) ? 1 << nodd : (nodd == 0 ? 1 : 0);
+          const float wgt = (nodd == 0) ? 1.f : (nodd == 1 ? 0.5f : (nodd == 2 ? 0.25f : 0.125f));
+          int c0[3];  // the block's first source, 2S - PAD/2, from the node's own coordinates
 #pragma unroll
-      for (int u = 0; u < (1 << D); ++u) {
-        qc[u] = -1;
-        Mc[u] = 0;
-        if (u < combos) {
-          int Ic[3], bit = 0;
+          for (int a = 0; a < D; ++a) c0[a] = ((J[a] + PAD) & ~3) / 2 - PAD / 2;
 #pragma unroll
-          for (int a = 0; a < D; ++a) {
-            int d = 0;
-            if (J[a] & 1) { d = ((u >> bit) & 1) ? 1 : -1; ++bit; }
-            Ic[a] = (J[a] - d) >> 1;
-          }
-          if (in_range_nodes<D>(Ic, Lc.n)) {
-            const int64_t qq = lidx<D>(Lc, Ic);
-            if (qq >= 0) {
-              qc[u] = (int)qq;
-              Mc[u] = nflag<D>(N, Lc.shift, Ic);
+          for (int u = 0; u < (1 << D); ++u) {
+            if (u < combos) {
+              int idx = 0, bit = 0;
+#pragma unroll
+              for (int a = 0; a < D; ++a) {
+                int d = 0;
+                if (J[a] & 1) { d = ((u >> bit) & 1) ? 1 : -1; ++bit; }
+                idx = idx * 3 + ((J[a] - d) >> 1) - c0[a];
+              }
+              const float4 vc = src[wv][sb][idx];
+              acc[0] += vc.x * wgt; acc[1] += vc.y * wgt; acc[2] += vc.z * wgt;
             }
           }
         }
-      }
-#pragma unroll
-      for (int u = 0; u < (1 << D); ++u) {
-        if (qc[u] >= 0 && f_valid(Mc[u])) {
-          const float4 vc = Lc.grid[qc[u]];
-          acc[0] += vc.x * wgt; acc[1] += vc.y * wgt; acc[2] += vc.z * wgt;
-        }
+        g.x = acc[0]; g.y = acc[1];
+        if (D == 3) g.z = acc[2];
+        Lf.grid[q] = g;
       }
     }
-    g.x = acc[0]; g.y = acc[1];
-    if (D == 3) g.z = acc[2];
-    Lf.grid[q] = g;
+    wave_sync_lds();  // the sources are restaged for the next blocks
   }
 }
 
