@@ -93,10 +93,15 @@ def test_kernel_stress_matches_oracle_and_polar(pkg, O, dim):
         F2 = F.copy()
         assert L.agimp_host_constitutive(ctypes.byref(cfg), _p(F2), _p(Jp), _p(tau), 0) == 0
         np.testing.assert_array_equal(F2, F)
-        np.testing.assert_allclose(tau, O.stress(F, mu, la), rtol=1e-9, atol=1e-7)
+        # 3D: the kernel's Jacobi sweeps stop at an off-diagonal residual of 1e-10 (its outputs
+        # are FP32, rounded at 6e-8), which the polar rotation carries into tau at up to 5e-10
+        # of tau's largest entry; 2D (closed form) stays at FP64 rounding
+        ref = O.stress(F, mu, la)
+        tol = (1e-13 if dim == 2 else 2e-9) * np.abs(ref).max() + 1e-7
+        np.testing.assert_allclose(tau, ref, rtol=0, atol=tol)
         R, _ = polar(F)
         J = np.linalg.det(F)
-        np.testing.assert_allclose(tau, 2 * mu * (F - R) @ F.T + la * J * (J - 1) * np.eye(dim), rtol=1e-9, atol=1e-7)
+        np.testing.assert_allclose(tau, 2 * mu * (F - R) @ F.T + la * J * (J - 1) * np.eye(dim), rtol=0, atol=tol)
 
 
 def test_plasticity_clamp_closed_form(pkg):
