@@ -2,7 +2,8 @@
 // the singular-value clamp plasticity, for one particle (device code).
 //
 // 3x3: symmetric Jacobi eigen-analysis of A = F^T F (cyclic sweeps with exact rotations,
-// Golub & Van Loan, "Matrix Computations", 8.5) gives V; B = F V is then reduced to upper
+// Golub & Van Loan, "Matrix Computations", 8.5, stopped at an off-diagonal residual of 1e-10)
+// gives V; B = F V is then reduced to upper
 // triangular form with three Givens rotations (the QR step of McAdams et al. 2011, "Computing
 // the SVD of 3x3 matrices with minimal branching"), which gives U and the singular values
 // as diag(R).  U and V are rotations, so the smallest singular value carries det F's sign
