@@ -1892,6 +1892,60 @@ __global__ void __launch_bounds__(256) k_prolong(const SceneDesc* __restrict__ D
   }
 }
 
+// G2P's gather: the quadratic B-spline weights of the particle at x on level Ll and the
+// velocity and velocity gradient sums over its 3^d stencil nodes (nv = sum w v, G = sum v (x)
+// grad w, in cell units)
+template <int D>
+__device__ __forceinline__ void g2p_gather(const LevelDesc& Ll, int pow2, const float* x, float* nv, float (*G)[3]) {
+  float w[3][3], dw[3][3];
+  int J[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float qx = scaled(x[a], Ll, pow2);
+    int b = (int)(qx + 0.5f) - 1;
+    bspline(qx - (float)(b + 1), w[a], dw[a]);
+    J[a] = b + PAD;
+  }
+  uint32_t ca[3][3];  // per-axis address terms of the stencil nodes
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ca[a][i] = bidx_axis<D>(Ll.NB, a, J[a] + i);
+  if constexpr (D == 2) {
+#pragma unroll
+    for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+      for (int i1 = 0; i1 < 3; ++i1) {
+        float4 gv = Ll.grid[ca[0][i0] + ca[1][i1]];
+        float wt = w[0][i0] * w[1][i1];
+        float g0 = dw[0][i0] * w[1][i1], g1 = w[0][i0] * dw[1][i1];
+        nv[0] += gv.x * wt; nv[1] += gv.y * wt;
+        G[0][0] += gv.x * g0; G[0][1] += gv.x * g1;
+        G[1][0] += gv.y * g0; G[1][1] += gv.y * g1;
+      }
+  } else {
+#pragma unroll
+    for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+      for (int i1 = 0; i1 < 3; ++i1)
+#pragma unroll
+        for (int i2 = 0; i2 < 3; ++i2) {
+          float4 gv = Ll.grid[ca[0][i0] + ca[1][i1] + ca[2][i2]];
+          float w01 = w[0][i0] * w[1][i1];
+          float wt = w01 * w[2][i2];
+          float g0 = dw[0][i0] * w[1][i1] * w[2][i2];
+          float g1 = w[0][i0] * dw[1][i1] * w[2][i2];
+          float g2 = w01 * dw[2][i2];
+          float vv[3] = {gv.x, gv.y, gv.z};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            nv[a] += vv[a] * wt;
+            G[a][0] += vv[a] * g0; G[a][1] += vv[a] * g1; G[a][2] += vv[a] * g2;
+          }
+        }
+  }
+}
+
 // adaptive_gimp.py:252-275 (+ plasticity and the next substep's stress); reads slot
 // perm[pos] of the current buffer, writes slot pos of the next buffer (re-sorting for free).
 template <int D>
@@ -1926,58 +1980,21 @@ __global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, 
     flv[pos] = -1;
     return;
   }
-  int l = level_of_gbid(T, g);
-  const LevelDesc Ll = T.lv[l];
-  float w[3][3], dw[3][3];
-  int J[3];
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    float qx = scaled(x[a], Ll, T.pow2);
-    int b = (int)(qx + 0.5f) - 1;
-    bspline(qx - (float)(b + 1), w[a], dw[a]);
-    J[a] = b + PAD;
-  }
-  uint32_t ca[3][3];  // per-axis address terms of the stencil nodes
-#pragma unroll
-  for (int a = 0; a < D; ++a)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) ca[a][i] = bidx_axis<D>(Ll.NB, a, J[a] + i);
+  const int l = level_of_gbid(T, g);
+  // particles are sorted by level, so a wavefront's level is nearly always one: then its
+  // descriptor comes from scalar loads (no per-lane load round before the gathers)
+  const int l0 = __builtin_amdgcn_readfirstlane(l);
   float nv[3] = {0.f, 0.f, 0.f}, G[3][3] = {};
-  if constexpr (D == 2) {
-#pragma unroll
-    for (int i0 = 0; i0 < 3; ++i0)
-#pragma unroll
-      for (int i1 = 0; i1 < 3; ++i1) {
-        float4 gv = Ll.grid[ca[0][i0] + ca[1][i1]];
-        float wt = w[0][i0] * w[1][i1];
-        float g0 = dw[0][i0] * w[1][i1], g1 = w[0][i0] * dw[1][i1];
-        nv[0] += gv.x * wt; nv[1] += gv.y * wt;
-        G[0][0] += gv.x * g0; G[0][1] += gv.x * g1;
-        G[1][0] += gv.y * g0; G[1][1] += gv.y * g1;
-      }
+  float sc;
+  if (__ballot(l != l0) == 0ull) {
+    const LevelDesc Ll = T.lv[l0];
+    g2p_gather<D>(Ll, T.pow2, x, nv, G);
+    sc = T.pow2 ? ph.dt * Ll.inv_dx : ph.dt / Ll.dx;  // dt (0.5/r)^d / dx, r = 0.5
   } else {
-#pragma unroll
-    for (int i0 = 0; i0 < 3; ++i0)
-#pragma unroll
-      for (int i1 = 0; i1 < 3; ++i1)
-#pragma unroll
-        for (int i2 = 0; i2 < 3; ++i2) {
-          float4 gv = Ll.grid[ca[0][i0] + ca[1][i1] + ca[2][i2]];
-          float w01 = w[0][i0] * w[1][i1];
-          float wt = w01 * w[2][i2];
-          float g0 = dw[0][i0] * w[1][i1] * w[2][i2];
-          float g1 = w[0][i0] * dw[1][i1] * w[2][i2];
-          float g2 = w01 * dw[2][i2];
-          float vv[3] = {gv.x, gv.y, gv.z};
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            nv[a] += vv[a] * wt;
-            G[a][0] += vv[a] * g0; G[a][1] += vv[a] * g1; G[a][2] += vv[a] * g2;
-          }
-        }
+    const LevelDesc Ll = T.lv[l];
+    g2p_gather<D>(Ll, T.pow2, x, nv, G);
+    sc = T.pow2 ? ph.dt * Ll.inv_dx : ph.dt / Ll.dx;
   }
-  // dt (0.5/r)^d / dx with r = 0.5 (a power-of-two dx makes the product exact)
-  float sc = T.pow2 ? ph.dt * Ll.inv_dx : ph.dt / Ll.dx;
   float Fo[3][3];
 #pragma unroll
   for (int i = 0; i < D; ++i)
