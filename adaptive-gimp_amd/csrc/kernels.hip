@@ -370,7 +370,7 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
   float x[3] = {0.f, 0.f, 0.f};
   if (k < n)
 #pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
+    for (int a = 0; a < D; ++a) x[a] = pload(pfield(P, a), (uint32_t)k * 4u);
   const int g = k < n ? P.g[k] : -1;
   const bool live = g >= 0 && g < T.L;
   const LevelDesc Lg = T.lv[live ? g : 0];
@@ -467,7 +467,7 @@ __global__ void k_activate_static(const SceneDesc* __restrict__ Dsc, int par) {
   const LevelDesc F = T.lv[T.L - 1];
   float x[3];
 #pragma unroll
-  for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
+  for (int a = 0; a < D; ++a) x[a] = pload(pfield(P, a), (uint32_t)k * 4u);
   int fb[3];
 #pragma unroll
   for (int a = 0; a < D; ++a) fb[a] = base_cell(x[a], F, T.pow2);
@@ -882,7 +882,7 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
   if (live) {
     float x[3];
 #pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = P.f[(size_t)a * P.stride + k];
+    for (int a = 0; a < D; ++a) x[a] = pload(pfield(P, a), (uint32_t)k * 4u);
     const int gp = sd.check ? P.g[k] : -1;  // level whose request cells are checked
     bool viol = false;
     // fl is the finest level with an activated cell among the 2^d; the checked level gp (if
@@ -1399,7 +1399,7 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
         k = perm[ck.y + i];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-          int b = base_cell(P.f[(size_t)(PL::X + a) * P.stride + k], Ll, T.pow2) + PAD - 4 * B[a];
+          int b = base_cell(pload(pfield(P, PL::X + a), (uint32_t)k * 4u), Ll, T.pow2) + PAD - 4 * B[a];
           lc = lc * 4 + b;
         }
       }
@@ -1438,20 +1438,21 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
       const int i = base + tid;
       const bool valid = i < ck.z;
       const int k = valid ? sidx[i] : 0;
+      const uint32_t ko = (uint32_t)k * 4u;  // byte offset of the slot in each field
       float x[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f}, Cm[3][3] = {}, tau[6] = {}, mp = 0.f;
       if (valid) {
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-          x[a] = P.f[(size_t)(PL::X + a) * P.stride + k];
-          v[a] = P.f[(size_t)(PL::V + a) * P.stride + k];
+          x[a] = pload(pfield(P, PL::X + a), ko);
+          v[a] = pload(pfield(P, PL::V + a), ko);
         }
 #pragma unroll
         for (int a = 0; a < D; ++a)
 #pragma unroll
-          for (int q = 0; q < D; ++q) Cm[a][q] = P.f[(size_t)(PL::C + a * D + q) * P.stride + k];
+          for (int q = 0; q < D; ++q) Cm[a][q] = pload(pfield(P, PL::C + a * D + q), ko);
 #pragma unroll
-        for (int q = 0; q < PL::NTAU; ++q) tau[q] = P.f[(size_t)(PL::TAU + q) * P.stride + k];
-        mp = P.f[(size_t)PL::M * P.stride + k];
+        for (int q = 0; q < PL::NTAU; ++q) tau[q] = pload(pfield(P, PL::TAU + q), ko);
+        mp = pload(pfield(P, PL::M), ko);
       }
       int lb[3] = {0, 0, 0}, key = 0;
       float w[3][3], dw[3][3], tt[3];
