@@ -873,7 +873,8 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
     const size_t st = (size_t)gridDim.x * blockDim.x;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < sd.w_flags; i += st) v[i] = a[i];
   }
-  // the grid-phase list counters (the prep pass read them; k_scompact appends from zero)
+  // the grid-phase list counters (the prep pass read them; the block-list compaction in the
+  // P2G launch appends from zero)
   if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
   if (blockIdx.x * blockDim.x >= (unsigned)n) return;  // whole workgroup past this scene
   int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1182,20 +1183,20 @@ __global__ void k_scatter_blocks(const SceneDesc* __restrict__ Dsc) {
   }
 }
 
-// compact the grid-phase block flags of every level into per-level lists
-template <int D>
-__global__ void __launch_bounds__(COMPACT_T) k_scompact(const SceneDesc* __restrict__ Dsc) {
-  AGIMP_SCENE
+// compact the grid-phase block flags of every level into per-level lists (workgroup bx of
+// COMPACT_T threads).  Only the grid phase reads the lists, and only the block flags that the
+// scatter sets feed them, so the workgroups run in the P2G launch, after its own.
+__device__ __forceinline__ void scompact_body(const Topo& T, int bx) {
   // each thread reads one word of 4 block flags (global block ids 4w .. 4w+3, levels laid out
   // one after the other); slots are counted per level in LDS and reserved with one atomic per
   // (workgroup, level)
   __shared__ int lcnt[MAXL], lbase[MAXL];
   const int n = T.total_blocks;
   const int nw = (n + 3) >> 2;
-  if (blockIdx.x * COMPACT_T >= (unsigned)nw) return;  // whole workgroup past this scene
+  if (bx * COMPACT_T >= nw) return;  // whole workgroup past this scene
   if (threadIdx.x < MAXL) lcnt[threadIdx.x] = 0;
   __syncthreads();
-  const int w = blockIdx.x * COMPACT_T + threadIdx.x;
+  const int w = bx * COMPACT_T + threadIdx.x;
   const uint32_t word = w < nw ? ((const uint32_t AGIMP_GLOBAL*)T.lv[0].sflag)[w] : 0u;
   int lv[4], off[4];
 #pragma unroll
@@ -1350,8 +1351,13 @@ constexpr int P2G_MAX_CH = 2048; // largest work chunk (particles of one leaf bl
 // kernel needs 62 either way.
 template <int D>
 __global__ void __launch_bounds__(P2G_T) __attribute__((amdgpu_waves_per_eu(5, 8)))
-k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
+k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par, int nwg) {
   AGIMP_SCENE
+  // workgroups past the first nwg build the grid-phase block lists
+  if ((int)blockIdx.x >= nwg) {
+    scompact_body(T, (int)blockIdx.x - nwg);
+    return;
+  }
   const PBuf& P = sd.P[par];
   int* perm = sd.S.perm;
   const int4* chunks = sd.S.chunks;
@@ -1379,7 +1385,7 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
   __shared__ int sidx[P2G_MAX_CH];
   const int nchunks = totals[1];
   const int tid = threadIdx.x;
-  for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
+  for (int c = blockIdx.x; c < nchunks; c += nwg) {
     const int4 ck = chunks[c];
     const int l = level_of_gbid(T, ck.x);
     const LevelDesc Ll = T.lv[l];
@@ -2225,12 +2231,15 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bo
     AGIMP_LAUNCH("k_scan_reduce", k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     AGIMP_LAUNCH("k_scan_final", k_scan_final, tg, dim3(SCAN_T), 0, s, d);
     AGIMP_LAUNCH("k_scatter_blocks", k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);
-    AGIMP_LAUNCH("k_scompact", k_scompact<DD>, cg, dim3(COMPACT_T), 0, s, d);
   });
 }
 
 void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par) {
-  AGIMP_DIM(m.dim, AGIMP_LAUNCH("k_p2g", k_p2g<DD>, grid2(m.p2g_grid, m.nscenes), dim3(P2G_T), 0, s, d, dt, par));
+  // P2G's workgroups, then those compacting the grid-phase block flags (COMPACT_T == P2G_T)
+  static_assert(COMPACT_T == P2G_T, "one workgroup size for both parts of the launch");
+  const int cw = (int)blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T);
+  AGIMP_DIM(m.dim, AGIMP_LAUNCH("k_p2g", k_p2g<DD>, grid2((size_t)m.p2g_grid + cw, m.nscenes), dim3(P2G_T), 0, s, d,
+                                dt, par, m.p2g_grid));
 }
 
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
