@@ -1788,6 +1788,95 @@ __global__ void __launch_bounds__(256) k_restrict(const SceneDesc* __restrict__ 
   }
 }
 
+// k_restrict with a workgroup per coarse leaf block (3D): the 9^3 fine nodes around the block
+// are staged three per thread, each node's flags and value loaded in one round without a
+// branch (the value whatever the flags say, a read of the dense grid), together with the
+// coarse nodes' flags and values; after one barrier the first wavefront sums its node's 3^d
+// entries in the same order as a direct gather.
+template <int D>
+__global__ void __launch_bounds__(256) k_restrict_wg(const SceneDesc* __restrict__ Dsc, int l, float dt_arg) {
+  AGIMP_SCENE
+  if (l >= T.L) return;
+  Physics ph = sd.ph;
+  if (dt_arg > 0.f) ph.dt = dt_arg;
+  const LevelDesc Lc = T.lv[l - 1];
+  const LevelDesc Lf = T.lv[l];
+  const NodeView N = node_view(T);
+  constexpr int R = 9, RN = (D == 2) ? R * R : R * R * R, NT = 256, NPT = (RN + NT - 1) / NT;
+  __shared__ float4 tile[RN];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int nblk = T.scount[l - 1];
+  for (int e = blockIdx.x; e < nblk; e += gridDim.x) {  // uniform per workgroup
+    const int b = __builtin_amdgcn_readfirstlane(Lc.slist[e]);
+    int B[3], f0[3];
+    block_coords<D>(Lc.NB, b, B);
+#pragma unroll
+    for (int a = 0; a < D; ++a) f0[a] = 2 * (4 * B[a] - PAD) - 1;  // first fine node (unpadded)
+    uint8_t M[NPT];
+    int nodd[NPT];
+    bool use[NPT];
+    float4 g[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int i = tid + NT * k;
+      int If[3], r = i;
+      nodd[k] = 0;
+#pragma unroll
+      for (int a = D - 1; a >= 0; --a) { If[a] = f0[a] + r % R; r /= R; nodd[k] += If[a] & 1; }
+      const int64_t q = lidx<D>(Lf, If);
+      use[k] = i < RN && in_range_nodes<D>(If, Lf.n) && q >= 0;
+      M[k] = nflag_nb<D>(N, Lf.shift, If);
+      g[k] = Lf.grid[use[k] ? (uint32_t)q : 0u];
+    }
+    // the first wavefront's coarse node (lanes < 4^d)
+    int I[3], w[3];
+    if (D == 2) { w[0] = (lane >> 2) & 3; w[1] = lane & 3; }
+    else { w[0] = lane >> 4; w[1] = (lane >> 2) & 3; w[2] = lane & 3; }
+#pragma unroll
+    for (int a = 0; a < D; ++a) I[a] = 4 * B[a] - PAD + w[a];
+    const bool mine = tid < Pow4<D>::v;
+    const uint8_t Mcl = nflag_nb<D>(N, Lc.shift, I);
+    const uint8_t Mc = (mine && in_range_nodes<D>(I, Lc.n)) ? Mcl : (uint8_t)0;
+    const uint32_t qc = (uint32_t)b * Pow4<D>::v + (lane & (Pow4<D>::v - 1));
+    const float4 gc = Lc.grid[qc];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+      const int i = tid + NT * k;
+      const bool u = use[k] && ((nodd[k] == 0 && f_act(M[k])) || f_T(M[k]) || f_ghost(M[k]));
+      const float wgt = (nodd[k] == 0) ? 1.f : (nodd[k] == 1 ? 0.5f : (nodd[k] == 2 ? 0.25f : 0.125f));
+      if (i < RN)
+        tile[i] = u ? make_float4(g[k].x * wgt, g[k].y * wgt, g[k].z * wgt, g[k].w * wgt)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    if (f_valid(Mc)) {
+      constexpr int NS = (D == 2) ? 9 : 27;
+      float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
+        int r = s2, idx = 0;
+        int rr[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) { rr[a] = 2 * w[a] + 1 - ((r % 3) - 1); r /= 3; }  // If - f0
+#pragma unroll
+        for (int a = 0; a < D; ++a) idx = idx * R + rr[a];
+        const float4 v = tile[idx];
+        acc.x += v.x; acc.y += v.y; acc.z += v.z; acc.w += v.w;
+      }
+      // no source contributed: the node keeps its value (adding zeros changes nothing);
+      // level 0 is complete after this restriction, so its grid update is applied here
+      const bool add = acc.x != 0.f || acc.y != 0.f || acc.z != 0.f || acc.w != 0.f;
+      if (add || l == 1) {
+        float4 gg = gc;
+        gg.x += acc.x; gg.y += acc.y; gg.z += acc.z; gg.w += acc.w;
+        const bool upd = (l == 1) && grid_update<D>(T, Lc, ph, I, Mc, gg);
+        if (add || upd) Lc.grid[qc] = gg;
+      }
+    }
+    __syncthreads();  // the tile is refilled for the next block
+  }
+}
+
 // The grid update of single-level scenes.  With two or more levels it is fused into the
 // last restriction (level 0) and into each prolongation (levels >= 1).
 template <int D>
@@ -2245,7 +2334,10 @@ void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int 
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
   const dim3 gg = grid2(m.grid_grid, m.nscenes);
   AGIMP_DIM(m.dim, {
-    for (int l = m.maxL - 1; l >= 1; --l) AGIMP_LAUNCH("k_restrict", k_restrict<DD>, gg, dim3(256), 0, s, d, l, dt);
+    for (int l = m.maxL - 1; l >= 1; --l) {
+      if constexpr (DD == 3) AGIMP_LAUNCH("k_restrict", k_restrict_wg<DD>, gg, dim3(256), 0, s, d, l, dt);
+      else AGIMP_LAUNCH("k_restrict", k_restrict<DD>, gg, dim3(256), 0, s, d, l, dt);
+    }
     if (m.any_single) AGIMP_LAUNCH("k_gridop", k_gridop<DD>, gg, dim3(256), 0, s, d, dt);
     for (int l = 1; l < m.maxL; ++l) AGIMP_LAUNCH("k_prolong", k_prolong<DD>, gg, dim3(256), 0, s, d, l, dt);
   });
