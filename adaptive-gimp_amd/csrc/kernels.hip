@@ -1508,52 +1508,58 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par, int nwg) {
             }
           }
       } else {
-        // p_a = mw (vb_a + C_a . i) + S_a . grad w, with mw = w0 w1 w2 m and grad w =
-        // (dw0 w1 w2, w0 dw1 w2, w0 w1 dw2), regrouped so that most of it is computed once per
-        // (i0, i1):  p_a = w2 F_a + w01 E_a(i2) + mw i2 C_a2, where
-        // F_a = w01 m (vb_a + C_a0 i0 + C_a1 i1) + S_a0 dw0 w1 + S_a1 w0 dw1, E_a(i2) = S_a2 dw2
+        // p_a = mw (vb_a + C_a . i) + S_a . grad w with w = N0 N1 N2 (N_a, D_a: the weights along
+        // axis a and their derivatives), regrouped by axis so that each node costs two fused
+        // multiply-adds per component:
+        //   p_a = N2 F_a(i0, i1) + N0 N1 G_a(i2),   F_a = N1 A_a(i0) + N0 B_a(i1),
+        //   A_a(i0) = m N0 (vb_a + C_a0 i0) + S_a0 D0,  B_a(i1) = m N1 C_a1 i1 + S_a1 D1,
+        //   G_a(i2) = m N2 C_a2 i2 + S_a2 D2
+        // (i C written as sums: the compiler may not drop a product with the constant 0)
         const SegOn on = seg_on(sm);
-        float E[3][3], C2[3];
+        float A[3][3], Bq[3][3], G[3][3];  // [i][a]
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          C2[a] = 2.0f * Cm[a][2];
+        for (int i = 0; i < 3; ++i) {
+          const float mN0 = mp * w[0][i], mN1 = mp * w[1][i], mN2 = mp * w[2][i];
 #pragma unroll
-          for (int i2 = 0; i2 < 3; ++i2) E[a][i2] = S[a][2] * dw[2][i2];
+          for (int a = 0; a < 3; ++a) {
+            const float c0 = i == 1 ? Cm[a][0] : Cm[a][0] + Cm[a][0];
+            const float c1 = i == 1 ? Cm[a][1] : Cm[a][1] + Cm[a][1];
+            const float c2 = i == 1 ? Cm[a][2] : Cm[a][2] + Cm[a][2];
+            A[i][a] = mN0 * (i == 0 ? vb[a] : vb[a] + c0) + S[a][0] * dw[0][i];
+            Bq[i][a] = i == 0 ? S[a][1] * dw[1][i] : mN1 * c1 + S[a][1] * dw[1][i];
+            G[i][a] = i == 0 ? S[a][2] * dw[2][i] : mN2 * c2 + S[a][2] * dw[2][i];
+          }
         }
+        // the wavefront's tile at the lane's base node: every node below is a constant offset
+        float4* tb = &tile[tid >> 6][(lb[0] * TS + lb[1]) * TS + lb[2]];
 #pragma unroll
         for (int i0 = 0; i0 < 3; ++i0)
 #pragma unroll
           for (int i1 = 0; i1 < 3; ++i1) {
-            const float w01 = w[0][i0] * w[1][i1];
-            const float m01 = w01 * mp;
-            const float h0 = dw[0][i0] * w[1][i1], h1 = w[0][i0] * dw[1][i1];
+            const float N01 = w[0][i0] * w[1][i1];
+            const float m01 = mp * N01;
             float F[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-              F[a] = m01 * (vb[a] + Cm[a][0] * i0 + Cm[a][1] * i1) + (S[a][0] * h0 + S[a][1] * h1);
+            for (int a = 0; a < 3; ++a) F[a] = w[1][i1] * A[i0][a] + w[0][i0] * Bq[i1][a];
 #pragma unroll
             for (int i2 = 0; i2 < 3; ++i2) {
               // the tile read is issued first (after the previous node's write, which it must
               // follow), so the node's arithmetic and scan hide its latency.  Every lane reads
               // and adds (its node lies in the tile; only tails write), and the sum is used on
-              // every path (the empty asm below): a read whose wait sits only in the tails'
-              // branch makes the next read wait for it again before reusing its registers.
-              float4& t = tile[tid >> 6][((lb[0] + i0) * TS + (lb[1] + i1)) * TS + (lb[2] + i2)];
-              float4 u = t;
+              // every path (the asm below): a read whose wait sits only in the tails' branch
+              // makes the next read wait for it again before reusing its registers.
+              float4* tn = tb + (i0 * TS + i1) * TS + i2;
+              float4 u = *tn;
               const float w2 = w[2][i2];
               float mw = m01 * w2;
               float p[3];
 #pragma unroll
-              for (int a = 0; a < 3; ++a) {
-                float pa = w2 * F[a] + w01 * E[a][i2];
-                if (i2 == 1) pa += mw * Cm[a][2];
-                if (i2 == 2) pa += mw * C2[a];
-                p[a] = pa;
-              }
+              for (int a = 0; a < 3; ++a) p[a] = w2 * F[a] + N01 * G[i2][a];
               seg_sum4_wave(p[0], p[1], p[2], mw, on);
               u.x += p[0]; u.y += p[1]; u.z += p[2]; u.w += mw;
-              if (writer) t = u;
-              asm volatile("" ::"v"(u.x), "v"(u.y), "v"(u.z), "v"(u.w));
+              if (writer) *tn = u;
+              // compiler barrier ("memory"): the next node's tile read stays after this write
+              asm volatile("" ::"v"(u.x), "v"(u.y), "v"(u.z), "v"(u.w) : "memory");
             }
           }
       }
