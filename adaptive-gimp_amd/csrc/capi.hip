@@ -617,8 +617,6 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   rc = rc ? rc : dalloc(h, &T.cflag, (flags + 3) & ~(size_t)3);
   rc = rc ? rc : dalloc(h, &T.clist, flags);
   rc = rc ? rc : dalloc(h, &T.ccount, 4);
-  // per listed cell, the coarse-fine faces the T marks found (read by the ghost marks)
-  rc = rc ? rc : dalloc(h, &T.fword, cells);
   if (rc) { agimp_destroy(h); return rc; }
   size_t co = 0, fo = 0;
   for (int l = 0; l < T.L; ++l) {

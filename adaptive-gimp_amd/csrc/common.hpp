@@ -65,7 +65,6 @@ struct Topo {
   uint8_t AGIMP_GLOBAL* cflag;    // [total_blocks] leaf block holds a non-zero cell (by global block id)
   int AGIMP_GLOBAL* clist;        // [total_blocks] compacted cflag (global block ids)
   int AGIMP_GLOBAL* ccount;       // [4] list lengths of the successive compactions of cflag
-  uint32_t AGIMP_GLOBAL* fword;   // [cells] per listed cell: parent level + 1 of each coarse-fine face
 };
 
 struct Physics {

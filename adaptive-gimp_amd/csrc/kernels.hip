@@ -647,72 +647,14 @@ __device__ __forceinline__ void face_parents(const Topo& T, int l, const int* I,
     if (lp[j] == l) lp[j] = -1;  // cell_mask[l][I1] == ACTIVATED
 }
 
-// adaptive_gimp.py:170-174 for all levels at once.  The coarse-fine faces found here are also
-// recorded, as one word per listed cell (3 bits per face: activated parent level + 1, or 0),
-// so that the ghost marks need not search again.
-template <int D>
-__global__ void k_mark_T(const SceneDesc* __restrict__ Dsc) {
-  AGIMP_SCENE
-  const int* list = T.clist;
-  const int* countp = T.ccount + 2;
-  unsigned long long* cnt = sd.cnt;
-  const int nlist = *countp;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
-  int l, I[3];
-  uint32_t q;
-  uint32_t word = 0;
-  if (decode_list_wave<D>(T, list, nlist, t, l, I, q) && T.lv[l].cell[q] == ACTIVATED) {
-    int lp[2 * D];
-    face_parents<D>(T, l, I, lp);
-    // the nodes of the face shared with neighbour I +- e_ax are corners of this cell (corner
-    // bit 1 on axis ax for the + side, 0 for the - side); their node-mask addresses on the
-    // finest level from per-axis terms
-    const LevelDesc Lf = T.lv[T.L - 1];
-    const int sh = T.lv[l].shift;
-    uint32_t ct[3][2];
-    bool cin[3][2];
-#pragma unroll
-    for (int a = 0; a < D; ++a)
-#pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int j = ((I[a] + k) << sh) + PAD;
-        cin[a][k] = j >= 0 && j < Lf.E;
-        ct[a][k] = bidx_axis<D>(Lf.NB, a, j);
-      }
-#pragma unroll
-    for (int j = 0; j < 2 * D; ++j) {
-      if (lp[j] < 0) continue;
-      word |= (uint32_t)(lp[j] + 1) << (3 * j);
-      const int ax = j >> 1, cb = 1 - (j & 1);  // the face's corner bit on axis ax
-      const int msk = (1 << (l - lp[j])) - 1;
-#pragma unroll
-      for (int u = 0; u < (1 << D); ++u) {
-        int kk[3];
-#pragma unroll
-        for (int a = 0; a < D; ++a) kk[a] = (u >> (D - 1 - a)) & 1;
-        if (kk[ax] != cb) continue;
-        bool any = false, in = true;
-        uint32_t qq = 0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          any = any || (((I[a] + kk[a]) & msk) != 0);
-          in = in && cin[a][kk[a]];
-          qq += ct[a][kk[a]];
-        }
-        if (!any) continue;
-        if (!in) { count(cnt, CNT_OOB); continue; }
-        // a T node lies on the face this cell shares with its neighbour, so it is a corner
-        // of this exactly-activated cell: k_node_A has already stored ACTIVATED there and
-        // flagged its grid-phase blocks, and ghost marks come later, so the byte holds A or
-        // A|T and |= T is the plain store of A|T (no read)
-        T.node[qq] = ACTIVATED | T_JUNCTION;
-      }
-    }
-  }
-  T.fword[t] = word;
-  }
-}
-
+// adaptive_gimp.py:170-174 (T-junction marks) and :176-183 (ghost marks) for all levels in
+// one pass over the listed cells.  The ghost marks may run beside the T marks: they touch only
+// cells that are not exactly ACTIVATED (every level in (lp, l] over the neighbour lies below
+// its finest activated ancestor, at lp), which the T marks' face searches compare with
+// ACTIVATED only, and only nodes still 0, while the T marks write nodes that the node A marks
+// (the previous pass) already set.  Each cell's coarse-fine faces (3 bits each: activated
+// parent level + 1, or 0) are found with its T marks, then each wavefront spreads the faces
+// of its 64 cells over its lanes for the ghost marks, 64 faces per round.
 // ghost marks of one coarse-fine face (neighbour I1 of level l, activated parent level lp)
 template <int D>
 __device__ __forceinline__ void mark_G_face(const Topo& T, unsigned long long* cnt, const int* I1, int l, int lp) {
@@ -739,23 +681,69 @@ __device__ __forceinline__ void mark_G_face(const Topo& T, unsigned long long* c
   }
 }
 
-// adaptive_gimp.py:176-183 for all levels at once, after every T mark, over the coarse-fine
-// faces the T marks recorded
 template <int D>
-__global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
+__global__ void k_mark_TG(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const int* list = T.clist;
-  const int* countp = T.ccount + 2;
   unsigned long long* cnt = sd.cnt;
-  const int nlist = *countp;
-  // Each wavefront loads the words of 64 consecutive listed cells, then spreads their recorded
-  // faces over its lanes, 64 faces per round (lane f takes the f-th face in cell order).  The
-  // loop is uniform over each workgroup, so every lane takes part in the shuffles.
+  const int nlist = T.ccount[2];
   const int N = nlist * Pow4<D>::v;
   const int lane = __lane_id();
+  // the loop is uniform over each workgroup, so every lane takes part in the shuffles
   for (int base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
     const int t = base + threadIdx.x;
-    const uint32_t word = t < N ? T.fword[t] : 0u;
+    int l = 0, I[3] = {0, 0, 0};
+    uint32_t q;
+    uint32_t word = 0;
+    if (decode_list_wave<D>(T, list, nlist, t, l, I, q) && T.lv[l].cell[q] == ACTIVATED) {
+      int lp[2 * D];
+      face_parents<D>(T, l, I, lp);
+      // the nodes of the face shared with neighbour I +- e_ax are corners of this cell (corner
+      // bit 1 on axis ax for the + side, 0 for the - side); their node-mask addresses on the
+      // finest level from per-axis terms
+      const LevelDesc Lf = T.lv[T.L - 1];
+      const int sh = T.lv[l].shift;
+      uint32_t ct[3][2];
+      bool cin[3][2];
+#pragma unroll
+      for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          const int j = ((I[a] + k) << sh) + PAD;
+          cin[a][k] = j >= 0 && j < Lf.E;
+          ct[a][k] = bidx_axis<D>(Lf.NB, a, j);
+        }
+#pragma unroll
+      for (int j = 0; j < 2 * D; ++j) {
+        if (lp[j] < 0) continue;
+        word |= (uint32_t)(lp[j] + 1) << (3 * j);
+        const int ax = j >> 1, cb = 1 - (j & 1);  // the face's corner bit on axis ax
+        const int msk = (1 << (l - lp[j])) - 1;
+#pragma unroll
+        for (int u = 0; u < (1 << D); ++u) {
+          int kk[3];
+#pragma unroll
+          for (int a = 0; a < D; ++a) kk[a] = (u >> (D - 1 - a)) & 1;
+          if (kk[ax] != cb) continue;
+          bool any = false, in = true;
+          uint32_t qq = 0;
+#pragma unroll
+          for (int a = 0; a < D; ++a) {
+            any = any || (((I[a] + kk[a]) & msk) != 0);
+            in = in && cin[a][kk[a]];
+            qq += ct[a][kk[a]];
+          }
+          if (!any) continue;
+          if (!in) { count(cnt, CNT_OOB); continue; }
+          // a T node lies on the face this cell shares with its neighbour, so it is a corner
+          // of this exactly-activated cell: the node A marks have stored ACTIVATED there and
+          // flagged its grid-phase blocks, and ghost marks never write a non-zero node, so the
+          // byte holds A or A|T and |= T is the plain store of A|T (no read)
+          T.node[qq] = ACTIVATED | T_JUNCTION;
+        }
+      }
+    }
+    // ghost marks of the wavefront's faces
     int nface = 0;
 #pragma unroll
     for (int j = 0; j < 2 * D; ++j) nface += ((word >> (3 * j)) & 7u) != 0u;
@@ -766,7 +754,6 @@ __global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
       if (lane >= o) incl += v;
     }
     const int total = __shfl(incl, 63);
-    const int t0 = t - lane;  // the wavefront's first cell
     for (int f0 = 0; f0 < total; f0 += 64) {
       const int f = f0 + lane;
       // source lane: the first with incl > f
@@ -778,6 +765,10 @@ __global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
       }
       const uint32_t w = __shfl(word, src);
       const int k = f - (__shfl(incl, src) - __shfl(nface, src));  // rank of the face in its cell
+      const int ls = __shfl(l, src);
+      int I1[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) I1[a] = __shfl(I[a], src);
       if (f >= total) continue;
       int j = 0, seen = 0;
 #pragma unroll
@@ -786,15 +777,9 @@ __global__ void k_mark_G(const SceneDesc* __restrict__ Dsc) {
         if (on && seen == k) j = jj;
         seen += on;
       }
-      const int lp = (int)((w >> (3 * j)) & 7u) - 1;
-      int l, I[3];
-      uint32_t q;
-      if (!decode_list<D>(T, list, nlist, t0 + src, l, I, q)) continue;
-      int I1[3];
-#pragma unroll
-      for (int a = 0; a < D; ++a) I1[a] = I[a];
+      const int lpj = (int)((w >> (3 * j)) & 7u) - 1;
       I1[j >> 1] += (j & 1) ? -1 : 1;
-      mark_G_face<D>(T, cnt, I1, l, lp);
+      mark_G_face<D>(T, cnt, I1, ls, lpj);
     }
   }
 }
@@ -2305,8 +2290,7 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
     }
     AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 2);
     AGIMP_LAUNCH("k_node_A", k_node_A<DD>, lg, dim3(256), 0, s, d);
-    AGIMP_LAUNCH("k_mark_T", k_mark_T<DD>, lg, dim3(256), 0, s, d);
-    AGIMP_LAUNCH("k_mark_G", k_mark_G<DD>, lg, dim3(256), 0, s, d);
+    AGIMP_LAUNCH("k_mark_TG", k_mark_TG<DD>, lg, dim3(256), 0, s, d);
     if (!chained) AGIMP_LAUNCH("k_copy_flags", k_copy_flags, fg, dim3(256), 0, s, d, 1);
     if (learn)
       for (int l = m.maxL - 2; l >= 0; --l) AGIMP_LAUNCH("k_cover", k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);

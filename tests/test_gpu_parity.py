@@ -354,7 +354,7 @@ def test_profile_report_lists_kernels_and_phases(pkg):
         sim.substep(wl.dt)
     kt = sim.kernel_timings()
     ph = sim.timings()
-    for k in ("k_p2g", "k_g2p", "k_levelmap", "k_activate_dynamic", "k_mark_T", "k_mark_G", "k_restrict", "k_prolong"):
+    for k in ("k_p2g", "k_g2p", "k_levelmap", "k_activate_dynamic", "k_mark_TG", "k_restrict", "k_prolong"):
         assert k in kt, (k, sorted(kt))
     assert kt["k_p2g"][1] == 2 and kt["k_g2p"][1] == 2
     assert kt["k_restrict"][1] == 2 * (wl.level - 1)
