@@ -2011,25 +2011,33 @@ __device__ __forceinline__ void g2p_gather(const LevelDesc& Ll, int pow2, const 
         G[1][0] += gv.y * g0; G[1][1] += gv.y * g1;
       }
   } else {
+    // regrouped by axis: for each (i0, i1) the sums along axis 2, s_a = sum N2 v_a and
+    // t_a = sum D2 v_a, then nv_a += N0 N1 s_a, G_a0 += D0 N1 s_a, G_a1 += N0 D1 s_a and
+    // G_a2 += N0 N1 t_a (w, dw: the weights N and their derivatives D per axis)
 #pragma unroll
     for (int i0 = 0; i0 < 3; ++i0)
 #pragma unroll
-      for (int i1 = 0; i1 < 3; ++i1)
+      for (int i1 = 0; i1 < 3; ++i1) {
+        float4 gv[3];
 #pragma unroll
-        for (int i2 = 0; i2 < 3; ++i2) {
-          float4 gv = Ll.grid[ca[0][i0] + ca[1][i1] + ca[2][i2]];
-          float w01 = w[0][i0] * w[1][i1];
-          float wt = w01 * w[2][i2];
-          float g0 = dw[0][i0] * w[1][i1] * w[2][i2];
-          float g1 = w[0][i0] * dw[1][i1] * w[2][i2];
-          float g2 = w01 * dw[2][i2];
-          float vv[3] = {gv.x, gv.y, gv.z};
+        for (int i2 = 0; i2 < 3; ++i2) gv[i2] = Ll.grid[ca[0][i0] + ca[1][i1] + ca[2][i2]];
+        float sv[3], tv[3];
+        sv[0] = w[2][0] * gv[0].x; sv[1] = w[2][0] * gv[0].y; sv[2] = w[2][0] * gv[0].z;
+        tv[0] = dw[2][0] * gv[0].x; tv[1] = dw[2][0] * gv[0].y; tv[2] = dw[2][0] * gv[0].z;
 #pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            nv[a] += vv[a] * wt;
-            G[a][0] += vv[a] * g0; G[a][1] += vv[a] * g1; G[a][2] += vv[a] * g2;
-          }
+        for (int i2 = 1; i2 < 3; ++i2) {
+          sv[0] += w[2][i2] * gv[i2].x; sv[1] += w[2][i2] * gv[i2].y; sv[2] += w[2][i2] * gv[i2].z;
+          tv[0] += dw[2][i2] * gv[i2].x; tv[1] += dw[2][i2] * gv[i2].y; tv[2] += dw[2][i2] * gv[i2].z;
         }
+        const float w01 = w[0][i0] * w[1][i1], h0 = dw[0][i0] * w[1][i1], h1 = w[0][i0] * dw[1][i1];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          nv[a] += w01 * sv[a];
+          G[a][0] += h0 * sv[a];
+          G[a][1] += h1 * sv[a];
+          G[a][2] += w01 * tv[a];
+        }
+      }
   }
 }
 
