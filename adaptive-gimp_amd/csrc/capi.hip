@@ -643,6 +643,8 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   SortBufs& S = sd.S;
   rc = rc ? rc : dalloc(h, &S.gb, stride);
   rc = rc ? rc : dalloc(h, &S.rk, stride);
+  rc = rc ? rc : dalloc(h, &S.lkey, stride);
+  rc = rc ? rc : dalloc(h, &S.pkey, stride);
   rc = rc ? rc : dalloc(h, &S.perm, stride);
   rc = rc ? rc : dalloc(h, &S.flv, stride);
   rc = rc ? rc : dalloc(h, &S.bcnt, T.total_blocks);

@@ -125,6 +125,8 @@ __device__ __forceinline__ void pstore(float AGIMP_GLOBAL* base, uint32_t off, f
 struct SortBufs {
   int AGIMP_GLOBAL* gb;       // [n]  global block id of each slot's particle (-1: frozen)
   int AGIMP_GLOBAL* rk;       // [n]  rank inside its block (or among frozen particles)
+  uint8_t AGIMP_GLOBAL* lkey; // [n]  cell of each slot's particle inside its leaf block (P2G sort key)
+  uint8_t AGIMP_GLOBAL* pkey; // [n]  the same key by sorted position
   int AGIMP_GLOBAL* perm;     // [n]  sorted position -> slot
   int AGIMP_GLOBAL* flv;      // [n]  fl_p of the particle now in each slot
   int AGIMP_GLOBAL* bcnt;     // [total_blocks] particles per block

@@ -956,6 +956,11 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
         int b = (D == 2) ? ((J[0] >> 2) * Lf.NB + (J[1] >> 2))
                          : (((J[0] >> 2) * Lf.NB + (J[1] >> 2)) * Lf.NB + (J[2] >> 2));
         gbid = Lf.gbase + b;
+        // the base cell inside the leaf block: the P2G sort key
+        int lc = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) lc = lc * 4 + (J[a] & 3);
+        sd.S.lkey[k] = (uint8_t)lc;
       }
     }
   }
@@ -1137,6 +1142,7 @@ __global__ void k_scatter_blocks(const SceneDesc* __restrict__ Dsc) {
     const int r = sd.S.rk[t];
     const int pos = (g >= 0) ? boff[g] + r : totals[0] + r;
     sd.S.perm[pos] = t;
+    if (g >= 0) sd.S.pkey[pos] = sd.S.lkey[t];
     if (t == 0) {
       unsigned long long* cnt = sd.cnt;
       cnt[CNT_ACTIVE] += (unsigned long long)totals[0];
@@ -1387,12 +1393,10 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par, int nwg) {
       bool valid = i < ck.z;
       int k = 0, lc = 0;
       if (valid) {
+        // the particle's cell inside the block, as the level map found it (its base cell at
+        // this level minus the block's origin)
         k = perm[ck.y + i];
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          int b = base_cell(pload(pfield(P, PL::X + a), (uint32_t)k * 4u), Ll, T.pow2) + PAD - 4 * B[a];
-          lc = lc * 4 + b;
-        }
+        lc = sd.S.pkey[ck.y + i];
       }
       myk[r] = k;
       mylc[r] = lc;
