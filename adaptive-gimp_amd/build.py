@@ -45,7 +45,7 @@ def p2g_tile_order_ok(asm: str) -> tuple[bool, str]:
     after the previous node's tile write in the compiled code (another lane's node can be the
     same one; a wavefront's LDS operations complete in issue order).  Checks that the kernel's
     ds_read_b128 / ds_write_b128 sequence is: the tile clear's writes, 27 read-write pairs, then
-    the 4 flush reads.  Returns (ok, the sequence)."""
+    only reads (the flush's, if it reads whole nodes).  Returns (ok, the sequence)."""
     import re
 
     names = [n for n in re.findall(r"^(_ZN5agimp\w+):", asm, re.M) if "k_p2gILi3" in n]
@@ -55,8 +55,8 @@ def p2g_tile_order_ok(asm: str) -> tuple[bool, str]:
     body = asm[start:asm.index(".Lfunc_end", start)]
     seq = "".join("R" if op == "ds_read_b128" else "W"
                   for op in re.findall(r"^\s+(ds_read_b128|ds_write_b128)\b", body, re.M))
-    m = re.fullmatch(r"W+((?:RW)+)(R+)", seq)
-    ok = bool(m) and len(m.group(1)) == 2 * 27 and len(m.group(2)) == 4
+    m = re.fullmatch(r"W+((?:RW)+)(R*)", seq)
+    ok = bool(m) and len(m.group(1)) == 2 * 27
     return ok, seq
 
 

@@ -1554,24 +1554,20 @@ k_p2g(const SceneDesc* __restrict__ Dsc, float dt_arg, int par, int nwg) {
       }
     }
     __syncthreads();
-    // 3. flush
-    for (int i = tid; i < TN; i += P2G_T) {
-      float4 val = tile[0][i];
+    // 3. flush: four lanes per node, one per component, so that a wavefront's float atomics
+    //    go to the consecutive words of 16 nodes (a few cache lines per instruction) rather
+    //    than to one component of 64 nodes
+    for (int i = tid; i < 4 * TN; i += P2G_T) {
+      const int nd = i >> 2, comp = i & 3;
+      float val = ((const float*)&tile[0][nd])[comp];
 #pragma unroll
-      for (int w = 1; w < NWV; ++w) {
-        const float4 o = tile[w][i];
-        val.x += o.x; val.y += o.y; val.z += o.z; val.w += o.w;
-      }
-      if (val.x == 0.f && val.y == 0.f && val.z == 0.f && val.w == 0.f) continue;
+      for (int w = 1; w < NWV; ++w) val += ((const float*)&tile[w][nd])[comp];
+      if (val == 0.f) continue;  // also the z slot in 2D, which stays 0
       int J[3];
-      if (D == 2) { J[0] = 4 * B[0] + i / TS; J[1] = 4 * B[1] + i % TS; }
-      else { J[0] = 4 * B[0] + i / (TS * TS); J[1] = 4 * B[1] + (i / TS) % TS; J[2] = 4 * B[2] + i % TS; }
+      if (D == 2) { J[0] = 4 * B[0] + nd / TS; J[1] = 4 * B[1] + nd % TS; }
+      else { J[0] = 4 * B[0] + nd / (TS * TS); J[1] = 4 * B[1] + (nd / TS) % TS; J[2] = 4 * B[2] + nd % TS; }
       if (!in_box<D>(Ll.E, J)) continue;
-      float4* dst = &Ll.grid[bidx<D>(Ll.NB, J)];
-      if (val.x != 0.f) unsafeAtomicAdd(&dst->x, val.x);
-      if (val.y != 0.f) unsafeAtomicAdd(&dst->y, val.y);
-      if (D == 3 && val.z != 0.f) unsafeAtomicAdd(&dst->z, val.z);
-      if (val.w != 0.f) unsafeAtomicAdd(&dst->w, val.w);
+      unsafeAtomicAdd((float*)&Ll.grid[bidx<D>(Ll.NB, J)] + comp, val);
     }
     __syncthreads();
   }
