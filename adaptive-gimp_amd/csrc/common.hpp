@@ -49,6 +49,11 @@ struct LevelDesc {
   int gbase;    // first global block id of this level
   float dx, inv_dx;
   uint8_t AGIMP_GLOBAL* cell;   // [nblocks * 4^d]
+  // [nblocks * 4^d] in the cells' layout: entry N is 1 if level-l node N (the low corner of
+  // cell N) is a corner of an exactly-ACTIVATED level-l cell.  A particle's 2^d cells at level
+  // l are the cells of which its base cell + 1 is a corner, so one entry answers whether one
+  // of them is exactly ACTIVATED (the level map's test).
+  uint8_t AGIMP_GLOBAL* corner;
   float4 AGIMP_GLOBAL* grid;    // [nblocks * 4^d]
   uint8_t AGIMP_GLOBAL* sflag;  // [nblocks]   block must be processed by the grid phase
   int AGIMP_GLOBAL* slist;      // [nblocks]   compacted sflag list

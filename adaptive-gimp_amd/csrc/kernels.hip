@@ -230,8 +230,9 @@ __device__ __forceinline__ void clear_list_grid(const Topo& T) {
   }
 }
 
-// Before refine: cell masks, node mask, block flags, list counters.  Only flagged blocks can
-// hold marks: every cell write flags its leaf block (cflag), and every node mark lies in a
+// Before refine: cell masks, corner maps, node mask, block flags, list counters.  Only flagged
+// blocks can hold marks: every cell or corner-map write flags its leaf block (cflag), and
+// every node mark lies in a
 // finest-level block whose grid-phase flag (sflag) is set (the node marks set it, it is saved
 // after refine and restored, never cleared, by bucketing).  One thread per global block
 // zeroes the block's cells if cflag is set and, for finest blocks with sflag set, its nodes;
@@ -255,12 +256,15 @@ __global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc, int chained) {
   const int nb = T.total_blocks, fbase = T.lv[T.L - 1].gbase;
   uint8_t AGIMP_GLOBAL* sflag = T.lv[0].sflag;
   uint32_t AGIMP_GLOBAL* cells = (uint32_t AGIMP_GLOBAL*)T.lv[0].cell;  // block g at word g * WPB
+  uint32_t AGIMP_GLOBAL* corners = (uint32_t AGIMP_GLOBAL*)T.lv[0].corner;  // the same layout
   uint32_t AGIMP_GLOBAL* nodes = (uint32_t AGIMP_GLOBAL*)T.node;
   const int st = gridDim.x * blockDim.x;
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nb; g += st) {
     if (T.cflag[g]) {
 #pragma unroll
       for (int w = 0; w < WPB; ++w) cells[(size_t)g * WPB + w] = 0u;
+#pragma unroll
+      for (int w = 0; w < WPB; ++w) corners[(size_t)g * WPB + w] = 0u;
       T.cflag[g] = 0;
     }
     if (sflag[g]) {
@@ -560,7 +564,41 @@ __global__ void k_split(const SceneDesc* __restrict__ Dsc) {
 }
 
 // ------------------------------------------------------------------------------ node marks
-// adaptive_gimp.py:142-153: corners of exactly-ACTIVATED cells get ACTIVATED
+// The level-l corner map of exactly-ACTIVATED level-l cell I: its 2^d corners inside
+// the padded box get 1, and the leaf blocks they reach beyond the cell's own get their cflag
+// (so the next rebuild's clear zeroes them; the lists of this rebuild are already built).
+template <int D>
+__device__ __forceinline__ void mark_level_corners(const Topo& T, int l, const int* I) {
+  const LevelDesc Ll = T.lv[l];
+  uint32_t t[3][2];
+  bool in1[3], cross[3];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const int j = I[a] + PAD;  // in [0, E): the cell lies in the box
+    t[a][0] = bidx_axis<D>(Ll.NB, a, j);
+    t[a][1] = bidx_axis<D>(Ll.NB, a, j + 1);
+    in1[a] = j + 1 < Ll.E;
+    cross[a] = (j & 3) == 3;
+  }
+#pragma unroll
+  for (int u = 0; u < (1 << D); ++u) {
+    bool in = true, other = false;
+    uint32_t qq = 0;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const int k = (u >> (D - 1 - a)) & 1;
+      in = in && (k == 0 || in1[a]);
+      other = other || (k == 1 && cross[a]);
+      qq += t[a][k];
+    }
+    if (!in) continue;
+    Ll.corner[qq] = 1;
+    if (other) T.cflag[Ll.gbase + (int)(qq / Pow4<D>::v)] = 1;
+  }
+}
+
+// adaptive_gimp.py:142-153: corners of exactly-ACTIVATED cells get ACTIVATED; also the
+// level-l corner maps of the level map
 template <int D>
 __global__ void k_node_A(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
@@ -583,6 +621,7 @@ __global__ void k_node_A(const SceneDesc* __restrict__ Dsc) {
     sel |= 1u << u;
   }
   mark_valid_corners<D>(T, l, I, sel);
+  mark_level_corners<D>(T, l, I);
   }
 }
 
@@ -835,15 +874,148 @@ __device__ __forceinline__ bool unit_covers(const Topo& T, int ulev, const int8_
 }
 
 // ------------------------------------------------------------------------------ bucketing
-constexpr int LM_TABLE_BITS = 8, LM_TABLE = 1 << LM_TABLE_BITS;  // >= threads per workgroup
+constexpr int LM_T = 256, LM_P = 1;  // level map: threads per workgroup, particles per thread
+constexpr int LM_TABLE_BITS = 8, LM_TABLE = 1 << LM_TABLE_BITS;  // >= particles per workgroup
+static_assert(LM_TABLE >= LM_T * LM_P, "one table entry per particle at most");
 constexpr int LM_EMPTY = -2;
+
 // adaptive_gimp.py:189-216: fl_p = finest level with an exactly-ACTIVATED cell among the
-// particle's 2^d cells; rank inside its (level, leaf block) bucket (run sums merged per workgroup).
+// particle's 2^d cells.  Those cells are the ones of which the base cell + 1 is a corner, so
+// per level one entry of the corner map answers.  lm_probe issues the loads of the entries of
+// all levels for LM_P particles before any is used (inb: the corner lies in the padded box); a
+// corner below the box has none of the cells in the box, and one beyond its high end (cells
+// partly outside the box) is left to lm_finish, which tests the cells in the box.
+template <int D, int NL>
+__device__ __forceinline__ void lm_probe(const Topo& T, const float (*x)[3], uint32_t (*hv)[NL], uint32_t* inb,
+                                         uint32_t* slow) {
+#pragma unroll
+  for (int p = 0; p < LM_P; ++p) inb[p] = slow[p] = 0;
+  // NL: the most levels of the scenes of the launch; a scene with fewer probes its coarsest
+  // level again and ignores the result (no branch that would split the loads)
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const bool has = T.L - 1 - i >= 0;
+    const int l = max(T.L - 1 - i, 0);
+    const LevelDesc Ll = T.lv[l];
+#pragma unroll
+    for (int p = 0; p < LM_P; ++p) {
+      int N[3], Nc[3];
+      bool lo = !has, hi = false;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        N[a] = base_cell(x[p][a], Ll, T.pow2) + 1 + PAD;
+        lo = lo || N[a] < 0;
+        hi = hi || N[a] >= Ll.E;
+        Nc[a] = min(max(N[a], 0), Ll.E - 1);
+      }
+      // every lane loads (a clamped address where the corner lies outside), and the byte stays
+      // in a register of its own until every level's load is issued: a select or a branch on
+      // it here would wait for it
+      hv[p][i] = Ll.corner[bidx<D>(Ll.NB, Nc)];
+      if (!lo && !hi) inb[p] |= 1u << i;
+      if (!lo && hi) slow[p] |= 1u << i;
+    }
+  }
+}
+
+// The rest of the level map for the particle in slot k at x: the levels left to a cell test,
+// fl, the refinement criterion check (when on), and the block.  Returns the global id of its
+// (level, leaf block), or -1 (frozen: no such level, or its stencil leaves the padded box);
+// stores its cell inside that block (the P2G sort key) and counts what the counters record.
+template <int D, int NL>
+__device__ __forceinline__ int lm_finish(const SceneDesc& sd, const PBuf& P, int k, const float* x, const uint32_t* hv,
+                                         uint32_t inb, uint32_t slow) {
+  const Topo& T = sd.T;
+  unsigned long long* cnt = sd.cnt;
+  int fl = -1, gbid = -1;
+  bool hit[NL];
+#pragma unroll
+  for (int i = 0; i < NL; ++i) hit[i] = ((inb >> i) & 1u) && hv[i] != 0u;
+  if (slow) {
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      if (!((slow >> i) & 1u)) continue;
+      const LevelDesc Ll = T.lv[T.L - 1 - i];
+      int b[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Ll, T.pow2) + PAD;
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        int J[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) J[a] = b[a] + ((u >> (D - 1 - a)) & 1);
+        if (in_box<D>(Ll.E, J) && Ll.cell[bidx<D>(Ll.NB, J)] == ACTIVATED) hit[i] = true;
+      }
+    }
+  }
+  const int gp = sd.check ? P.g[k] : -1;  // level whose request cells are checked
+  bool viol = false;
+#pragma unroll
+  for (int i = 0; i < NL; ++i) {
+    const int l = T.L - 1 - i;
+    if (l < 0) continue;
+    if (hit[i] && fl < 0) fl = l;
+    if (l == gp) {
+      // the criterion check: the in-box request cells of level gp must hold ACTIVATED or
+      // COVERED, or lie in a unit predicted at gp or finer
+      const LevelDesc Ll = T.lv[l];
+      int b[3];
+#pragma unroll
+      for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], Ll, T.pow2);
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        int I[3], J[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
+          J[a] = I[a] + PAD;
+        }
+        if (in_box<D>(Ll.E, J) && !(Ll.cell[bidx<D>(Ll.NB, J)] & (ACTIVATED | COVERED)))
+          viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
+      }
+    }
+  }
+  if (viol) {
+    count(cnt, CNT_VIOL);
+    atomicMin(&cnt[CNT_FIRSTVIOL], cnt[CNT_STEP]);
+  }
+  if (fl >= 0) {
+    const LevelDesc Lf = T.lv[fl];
+    int J[3];
+    bool ok = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      J[a] = base_cell(x[a], Lf, T.pow2) + PAD;
+      ok = ok && J[a] >= 0 && J[a] + 2 < Lf.E;
+    }
+    if (!ok) {
+      count(cnt, CNT_OOB);
+    } else {
+      int b = (D == 2) ? ((J[0] >> 2) * Lf.NB + (J[1] >> 2))
+                       : (((J[0] >> 2) * Lf.NB + (J[1] >> 2)) * Lf.NB + (J[2] >> 2));
+      gbid = Lf.gbase + b;
+      // the base cell inside the leaf block: the P2G sort key
+      int lc = 0;
+#pragma unroll
+      for (int a = 0; a < D; ++a) lc = lc * 4 + (J[a] & 3);
+      sd.S.lkey[k] = (uint8_t)lc;
+    }
+  }
+  return gbid;
+}
+
+// Level mapping and the rank of every particle inside its (level, leaf block) bucket.  Each
+// thread takes LM_P particles (slots a workgroup's LM_T apart, so each wavefront's 64 lanes
+// hold consecutive slots), all loaded before any is used.  Ranks: runs of equal blocks inside
+// each wavefront are merged per workgroup in an LDS table (local offsets from LDS atomics),
+// then one global atomic per (workgroup, block) reserves the workgroup's range: consecutive
+// wavefronts mostly fill the same block, and one returning global atomic per run made them
+// queue on its counter.
 // save_flags (chained after a rebuild): also save the octree's grid-phase block flags (the
 // chunk pass adds to them next), the part of the bucketing preparation that must follow the
 // rebuild's node marks.
-template <int D>
-__global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_flags) {
+template <int D, int NL>
+__global__ void __launch_bounds__(LM_T) k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_flags) {
   AGIMP_SCENE
   const PBuf& P = sd.P[par];
   const int n = sd.n;
@@ -851,7 +1023,6 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
   int* rk = sd.S.rk;
   int* bcnt = sd.S.bcnt;
   int* frozen = sd.S.frozen;
-  unsigned long long* cnt = sd.cnt;
   if (save_flags) {
     const uint32_t* a = (const uint32_t*)T.lv[0].sflag;
     uint32_t* v = sd.vflag;
@@ -861,149 +1032,72 @@ __global__ void k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_
   // the grid-phase list counters (the prep pass read them; the block-list compaction in the
   // P2G launch appends from zero)
   if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
-  if (blockIdx.x * blockDim.x >= (unsigned)n) return;  // whole workgroup past this scene
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  bool live = k < n;
-  int fl = -1, gbid = -1;
-  if (live) {
-    float x[3];
-#pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = pload(pfield(P, a), (uint32_t)k * 4u);
-    const int gp = sd.check ? P.g[k] : -1;  // level whose request cells are checked
-    bool viol = false;
-    // fl is the finest level with an activated cell among the 2^d; the checked level gp (if
-    // any) needs its cells too.  The cells of all levels (unrolled over the level bound) are
-    // loaded first, all together, then examined.
-    constexpr int NC = 1 << D;
-    uint8_t c[MAXL][NC];
-    uint32_t inb[MAXL];  // bit u: cell u lies in the padded box
-    // the position is consumed here, on the one path into the level loop: otherwise the
-    // level loop's branches leave its load pending at their joins, and the wait for it there
-    // (the memory counter is in order) would also wait for the previous level's cells
-    asm volatile("" ::"v"(x[0]), "v"(x[1]), "v"(x[2]));
-#pragma unroll
-    for (int i = 0; i < MAXL; ++i) {
-      const int l = T.L - 1 - i;
-      inb[i] = 0;
-#pragma unroll
-      for (int u = 0; u < NC; ++u) c[i][u] = 0;
-      if (l < 0) continue;
-      const LevelDesc Ll = T.lv[l];
-      // per axis: the two cells' address terms and whether each lies in the padded box (the
-      // block-major index is a sum of per-axis terms, so the 2^d cells need 2d terms)
-      uint32_t t0[3], t1[3];
-      bool in0[3], in1[3];
-#pragma unroll
-      for (int a = 0; a < D; ++a) {
-        const int j = base_cell(x[a], Ll, T.pow2) + PAD;
-        in0[a] = j >= 0 && j < Ll.E;
-        in1[a] = j + 1 >= 0 && j + 1 < Ll.E;
-        t0[a] = bidx_axis<D>(Ll.NB, a, j);
-        t1[a] = bidx_axis<D>(Ll.NB, a, j + 1);
-      }
-#pragma unroll
-      for (int u = 0; u < NC; ++u) {
-        bool in = true;
-        uint32_t q = 0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-          const bool hi = (u >> (D - 1 - a)) & 1;
-          in = in && (hi ? in1[a] : in0[a]);
-          q += hi ? t1[a] : t0[a];
-        }
-        if (in) { c[i][u] = Ll.cell[q]; inb[i] |= 1u << u; }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < MAXL; ++i) {
-      const int l = T.L - 1 - i;
-      if (l < 0) continue;
-      bool hit = false;
-#pragma unroll
-      for (int u = 0; u < NC; ++u) hit = hit || c[i][u] == ACTIVATED;
-      if (hit && fl < 0) fl = l;
-      if (l == gp) {
-        int b[3];
-#pragma unroll
-        for (int a = 0; a < D; ++a) b[a] = base_cell(x[a], T.lv[l], T.pow2);
-#pragma unroll
-        for (int u = 0; u < NC; ++u)
-          if (((inb[i] >> u) & 1u) && !(c[i][u] & (ACTIVATED | COVERED))) {
-            int I[3];
-#pragma unroll
-            for (int a = 0; a < D; ++a) I[a] = b[a] + ((u >> (D - 1 - a)) & 1);
-            viol = viol || !unit_covers<D>(T, sd.ulev, sd.upred, l, I);
-          }
-      }
-    }
-    if (viol) {
-      count(cnt, CNT_VIOL);
-      atomicMin(&cnt[CNT_FIRSTVIOL], cnt[CNT_STEP]);
-    }
-    if (fl >= 0) {
-      const LevelDesc Lf = T.lv[fl];
-      int J[3];
-      bool ok = true;
-#pragma unroll
-      for (int a = 0; a < D; ++a) {
-        J[a] = base_cell(x[a], Lf, T.pow2) + PAD;
-        ok = ok && J[a] >= 0 && J[a] + 2 < Lf.E;
-      }
-      if (!ok) {
-        count(cnt, CNT_OOB);
-        fl = -1;
-      } else {
-        int b = (D == 2) ? ((J[0] >> 2) * Lf.NB + (J[1] >> 2))
-                         : (((J[0] >> 2) * Lf.NB + (J[1] >> 2)) * Lf.NB + (J[2] >> 2));
-        gbid = Lf.gbase + b;
-        // the base cell inside the leaf block: the P2G sort key
-        int lc = 0;
-#pragma unroll
-        for (int a = 0; a < D; ++a) lc = lc * 4 + (J[a] & 3);
-        sd.S.lkey[k] = (uint8_t)lc;
-      }
-    }
-  }
-  // rank inside the block: runs of equal blocks inside each wavefront are merged per
-  // workgroup in an LDS table (local offsets from LDS atomics), then one global atomic per
-  // (workgroup, block) reserves the workgroup's range: consecutive wavefronts mostly fill the
-  // same block, and one returning global atomic per run made them queue on its counter
+  const int k0 = blockIdx.x * (LM_T * LM_P) + threadIdx.x;
+  if (k0 - (int)threadIdx.x >= n) return;  // whole workgroup past this scene
   __shared__ int tkey[LM_TABLE], tcnt[LM_TABLE], tbase[LM_TABLE];
-  for (int i = threadIdx.x; i < LM_TABLE; i += blockDim.x) {
+  for (int i = threadIdx.x; i < LM_TABLE; i += LM_T) {
     tkey[i] = LM_EMPTY;
     tcnt[i] = 0;
   }
+  float x[LM_P][3];
+#pragma unroll
+  for (int p = 0; p < LM_P; ++p) {
+    const int k = k0 + p * LM_T;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) x[p][a] = 0.f;
+    if (k < n)
+#pragma unroll
+      for (int a = 0; a < D; ++a) x[p][a] = pload(pfield(P, a), (uint32_t)k * 4u);
+  }
+  // the positions are consumed here, on the one path into the level loop: otherwise its
+  // branches leave their loads pending at the joins
+#pragma unroll
+  for (int p = 0; p < LM_P; ++p) asm volatile("" ::"v"(x[p][0]), "v"(x[p][1]), "v"(x[p][2]));
+  uint32_t hv[LM_P][NL], inb[LM_P], slow[LM_P];
+  lm_probe<D, NL>(T, x, hv, inb, slow);
+  int key[LM_P];  // block id, -1 frozen, LM_EMPTY past the end
+#pragma unroll
+  for (int p = 0; p < LM_P; ++p) {
+    const int k = k0 + p * LM_T;
+    key[p] = k < n ? lm_finish<D, NL>(sd, P, k, x[p], hv[p], inb[p], slow[p]) : LM_EMPTY;
+    if (k < n) gb[k] = key[p];
+  }
   __syncthreads();
   const int lane = __lane_id();
-  const int key = live ? gbid : LM_EMPTY;  // -1: frozen
-  const int prev = __shfl_up(key, 1);
-  const bool head = !live || lane == 0 || prev != key;
-  const unsigned long long heads = __ballot(head);
   const unsigned long long le = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-  const int h = 63 - __clzll(heads & le);
-  const unsigned long long above = heads & ~((h == 63) ? ~0ull : ((1ull << (h + 1)) - 1ull));
-  const int nh = above ? (__ffsll((long long)above) - 1) : 64;
-  int slot = 0, loc = 0;
-  if (live && lane == h) {
-    const uint32_t hs = ((uint32_t)(key + 1) * 2654435761u) >> (32 - LM_TABLE_BITS);
-    for (int probe = 0; probe < LM_TABLE; ++probe) {
-      const int e = (int)((hs + probe) & (LM_TABLE - 1));
-      const int old = atomicCAS(&tkey[e], LM_EMPTY, key);
-      if (old == LM_EMPTY || old == key) { slot = e; break; }
+  int slot[LM_P], off[LM_P];
+#pragma unroll
+  for (int p = 0; p < LM_P; ++p) {
+    const bool live = key[p] != LM_EMPTY;
+    const int prev = __shfl_up(key[p], 1);
+    const bool head = !live || lane == 0 || prev != key[p];
+    const unsigned long long heads = __ballot(head);
+    const int h = 63 - __clzll(heads & le);
+    const unsigned long long above = heads & ~((h == 63) ? ~0ull : ((1ull << (h + 1)) - 1ull));
+    const int nh = above ? (__ffsll((long long)above) - 1) : 64;
+    int sl = 0, loc = 0;
+    if (live && lane == h) {
+      const uint32_t hs = ((uint32_t)(key[p] + 1) * 2654435761u) >> (32 - LM_TABLE_BITS);
+      for (int probe = 0; probe < LM_TABLE; ++probe) {
+        const int e = (int)((hs + probe) & (LM_TABLE - 1));
+        const int old = atomicCAS(&tkey[e], LM_EMPTY, key[p]);
+        if (old == LM_EMPTY || old == key[p]) { sl = e; break; }
+      }
+      loc = atomicAdd(&tcnt[sl], nh - h);
     }
-    loc = atomicAdd(&tcnt[slot], nh - h);
+    slot[p] = __shfl(sl, h);
+    off[p] = __shfl(loc, h) + (lane - h);
   }
-  slot = __shfl(slot, h);
-  loc = __shfl(loc, h);
   __syncthreads();
-  for (int i = threadIdx.x; i < LM_TABLE; i += blockDim.x) {
+  for (int i = threadIdx.x; i < LM_TABLE; i += LM_T) {
     const int kk = tkey[i];
     if (kk != LM_EMPTY) tbase[i] = atomicAdd(kk >= 0 ? &bcnt[kk] : frozen, tcnt[i]);
   }
   __syncthreads();
-  if (live) {
-    gb[k] = gbid;
-    rk[k] = tbase[slot] + loc + (lane - h);
+#pragma unroll
+  for (int p = 0; p < LM_P; ++p) {
+    const int k = k0 + p * LM_T;
+    if (key[p] != LM_EMPTY) rk[k] = tbase[slot[p]] + off[p];
   }
 }
 
@@ -2314,7 +2408,18 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bo
   const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {
     if (!chained) AGIMP_LAUNCH("k_bucket_prep", k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
-    AGIMP_LAUNCH("k_levelmap", k_levelmap<DD>, pg, dim3(256), 0, s, d, par, chained ? 1 : 0);
+    {
+      const dim3 lmg = grid2(blocks_for(m.max_n, LM_T * LM_P), m.nscenes);
+      const int sv = chained ? 1 : 0;
+      switch (m.maxL) {  // the level loop unrolled for the most levels of the launch's scenes
+        case 1: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 1>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
+        case 2: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 2>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
+        case 3: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 3>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
+        case 4: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 4>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
+        case 5: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 5>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
+        default: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, MAXL>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
+      }
+    }
     AGIMP_LAUNCH("k_scan_reduce", k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
     AGIMP_LAUNCH("k_scan_final", k_scan_final, tg, dim3(SCAN_T), 0, s, d);
     AGIMP_LAUNCH("k_scatter_blocks", k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);

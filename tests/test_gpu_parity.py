@@ -239,6 +239,48 @@ def test_wall_band_and_negative_base_cells(pkg, O):
     sim.close()
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_particles_at_the_high_edge_of_the_padded_boxes(pkg, O, dim):
+    """Particles whose 2^d cells at some level lie partly beyond the high end of that level's
+    padded box (their common corner is outside it, so the level map tests the cells in the
+    box one by one), next to particles that activate the last cells inside the box: fl_p, the
+    out-of-range and frozen counts and the octree must equal the oracle's."""
+    L, c, PAD = 3, 16, 4
+    rng = np.random.default_rng(11)
+    nb = 3000
+    xs = [rng.uniform(0.1, 0.9, (nb, dim))]
+    gs = [rng.integers(0, L, nb)]
+    for l in range(L):
+        n_l = c << l
+        e = 4 * ((n_l // 4 + 1 + 3) // 4 * 4)
+        E = e + 2 * PAD
+        for off, g in ((-1.0, l), (0.0, l), (0.0, L - 1), (0.3, 0)):  # base cell + PAD = E-2, E-1, E-1, E-1
+            for axes in ([0], [dim - 1], list(range(dim))):
+                k = 40
+                x = rng.uniform(0.3, 0.7, (k, dim))
+                for a in axes:
+                    x[:, a] = (E - PAD + off) / n_l + rng.uniform(-0.1, 0.1, k) / n_l
+                xs.append(x)
+                gs.append(np.full(k, g))
+    x = np.concatenate(xs)
+    n = x.shape[0]
+    st = dict(x=x, v=rng.standard_normal((n, dim)) * 0.1, F=np.tile(np.eye(dim), (n, 1, 1)), C=np.zeros((n, dim, dim)),
+              m=np.ones(n), g=np.concatenate(gs).astype(np.int32))
+    sim, ora = make_pair(pkg, O, dim, L, c, st, {})
+    for _ in range(2):
+        sim.substep(1e-5)
+        ora.substep(1e-5)
+    cnt = sim.counters()
+    assert cnt["oob"] == ora.oob() and cnt["oob"] > 0
+    r = ora.get_particles()
+    fl = sim.fl_p.to_numpy()
+    assert np.array_equal(fl, r["fl"]), f"{(fl != r['fl']).sum()} fl_p differ"
+    assert (fl == -1).sum() > 0
+    assert_topology_equal(sim, ora, O, L)
+    assert_particles_close(particle_errors(sim, ora))
+    sim.close()
+
+
 def test_single_particle(pkg, O):
     st = dict(x=np.array([[0.5, 0.5, 0.5]]), v=np.array([[1.0, 0.0, 0.0]]), F=np.eye(3)[None], C=np.zeros((1, 3, 3)),
               m=np.ones(1), g=np.array([2], np.int32))
