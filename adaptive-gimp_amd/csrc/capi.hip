@@ -334,13 +334,15 @@ hipError_t capture_chain(hipStream_t s, const SceneDesc* dd, const Dims& m, int 
   cap.release();
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
+  // inside the chain each G2P launch also clears its octree for the next substep's rebuild;
+  // the last keeps it (the caller may read it), and the first rebuild clears for itself
   for (int i = 0; i < SUBSTEP_CHAIN; ++i) {
     const int p = par ^ (i & 1);
-    launch_refine(s, dd, m, p, nullptr, true);
+    launch_refine(s, dd, m, p, nullptr, true, i > 0);
     launch_bucket(s, dd, m, p, true);
     launch_p2g(s, dd, m, 0.f, p);
     launch_grid(s, dd, m, 0.f);
-    launch_g2p(s, dd, m, 0.f, p);
+    launch_g2p(s, dd, m, 0.f, p, i + 1 < SUBSTEP_CHAIN);
   }
   hipError_t le = hipGetLastError();
   e = hipStreamEndCapture(s, &cap.graph);

@@ -202,20 +202,23 @@ __device__ __forceinline__ void mark_valid_corners(const Topo& T, int l, const i
   }
 }
 
-// zero the per-substep buffers of every scene in one launch (one grid-stride loop per buffer)
-__device__ __forceinline__ void fill_zero(uint32_t* p, size_t n) {
-  size_t st = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += st) p[i] = 0u;
+// Clears done by a part of a launch: workgroup wg of the nwg workgroups that take part
+// (grid-stride loops over those alone).
+
+// zero n words
+__device__ __forceinline__ void fill_zero(uint32_t* p, size_t n, int wg, int nwg) {
+  const size_t st = (size_t)nwg * blockDim.x;
+  for (size_t i = wg * (size_t)blockDim.x + threadIdx.x; i < n; i += st) p[i] = 0u;
 }
 
 // zero the grid of every block in the grid-phase lists (counts and lists as left by the
 // previous substep); grid-stride over all listed nodes
 template <int D>
-__device__ __forceinline__ void clear_list_grid(const Topo& T) {
+__device__ __forceinline__ void clear_list_grid(const Topo& T, int wg, int nwg) {
   size_t total = 0;
   for (int l = 0; l < T.L; ++l) total += (size_t)T.scount[l] * Pow4<D>::v;
-  const size_t st = (size_t)gridDim.x * blockDim.x;
-  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += st) {
+  const size_t st = (size_t)nwg * blockDim.x;
+  for (size_t t = wg * (size_t)blockDim.x + threadIdx.x; t < total; t += st) {
     size_t acc = 0;
     for (int l = 0; l < T.L; ++l) {
       const size_t cnt = (size_t)T.scount[l] * Pow4<D>::v;
@@ -230,36 +233,23 @@ __device__ __forceinline__ void clear_list_grid(const Topo& T) {
   }
 }
 
-// Before refine: cell masks, corner maps, node mask, block flags, list counters.  Only flagged
-// blocks can hold marks: every cell or corner-map write flags its leaf block (cflag), and
-// every node mark lies in a
-// finest-level block whose grid-phase flag (sflag) is set (the node marks set it, it is saved
-// after refine and restored, never cleared, by bucketing).  One thread per global block
-// zeroes the block's cells if cflag is set and, for finest blocks with sflag set, its nodes;
-// then both flags.  A thread reads and clears only its own block's flags.
-// chained (the bucketing pass follows in the same graph): also zero the grid of the previous
-// grid-phase lists (k_clear_sgrid's work) and leave their counters, which the bucketing pass
-// then neither re-reads for its clear nor needs; the level map resets them.
-// chained: it also does the bucketing pass's preparation that does not depend on the rebuild
-// (zero the per-block particle counters and the frozen counter, count the substep); the level
-// map saves the octree's grid-phase block flags.
+// The octree's marks, before a rebuild: cell masks, corner maps, node mask, block flags, the
+// compaction counters.  Only flagged blocks can hold marks: every cell or corner-map write
+// flags its leaf block (cflag), and every node mark lies in a finest-level block whose
+// grid-phase flag (sflag) is set (the node marks set it, it is saved after refine and
+// restored, never cleared, by bucketing).  One thread per global block zeroes the block's
+// cells and corners if cflag is set and, for finest blocks with sflag set, its nodes; then both
+// flags.  A thread reads and clears only its own block's flags.
 template <int D>
-__global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc, int chained) {
-  AGIMP_SCENE
-  if (chained) {
-    clear_list_grid<D>(T);
-    fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
-    fill_zero((uint32_t*)sd.S.frozen, 1);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
-  }
+__device__ __forceinline__ void clear_octree(const Topo& T, int wg, int nwg) {
   constexpr int WPB = Pow4<D>::v / 4;  // 32-bit words per leaf block of byte masks
   const int nb = T.total_blocks, fbase = T.lv[T.L - 1].gbase;
   uint8_t AGIMP_GLOBAL* sflag = T.lv[0].sflag;
   uint32_t AGIMP_GLOBAL* cells = (uint32_t AGIMP_GLOBAL*)T.lv[0].cell;  // block g at word g * WPB
   uint32_t AGIMP_GLOBAL* corners = (uint32_t AGIMP_GLOBAL*)T.lv[0].corner;  // the same layout
   uint32_t AGIMP_GLOBAL* nodes = (uint32_t AGIMP_GLOBAL*)T.node;
-  const int st = gridDim.x * blockDim.x;
-  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nb; g += st) {
+  const int st = nwg * blockDim.x;
+  for (int g = wg * blockDim.x + threadIdx.x; g < nb; g += st) {
     if (T.cflag[g]) {
 #pragma unroll
       for (int w = 0; w < WPB; ++w) cells[(size_t)g * WPB + w] = 0u;
@@ -275,8 +265,32 @@ __global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc, int chained) {
       sflag[g] = 0;
     }
   }
+  if (wg == 0 && threadIdx.x < 4) T.ccount[threadIdx.x] = 0;
+}
+
+// the bucketing pass's preparation that does not depend on the rebuild: zero the per-block
+// particle counters and the frozen counter, count the substep
+__device__ __forceinline__ void clear_bucket_counts(const SceneDesc& sd, int wg, int nwg) {
+  fill_zero((uint32_t*)sd.S.bcnt, (size_t)sd.T.total_blocks, wg, nwg);
+  fill_zero((uint32_t*)sd.S.frozen, 1, wg, nwg);
+  if (wg == 0 && threadIdx.x == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
+}
+
+// Before refine: the octree's marks (clear_octree).
+// chained (the bucketing pass follows in the same graph): also zero the grid of the previous
+// grid-phase lists (k_clear_sgrid's work) and leave their counters, which the bucketing pass
+// then neither re-reads for its clear nor needs; the level map resets them.  It also does the
+// bucketing pass's preparation that does not depend on the rebuild (clear_bucket_counts); the
+// level map saves the octree's grid-phase block flags.
+template <int D>
+__global__ void k_clear_refine(const SceneDesc* __restrict__ Dsc, int chained) {
+  AGIMP_SCENE
+  if (chained) {
+    clear_list_grid<D>(T, blockIdx.x, gridDim.x);
+    clear_bucket_counts(sd, blockIdx.x, gridDim.x);
+  }
+  clear_octree<D>(T, blockIdx.x, gridDim.x);
   if (!chained && blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < 4) T.ccount[threadIdx.x] = 0;
 }
 
 // the valid-node block flags of the octree: save after refine (to_saved) or restore
@@ -338,7 +352,7 @@ __global__ void __launch_bounds__(COMPACT_T) k_compact_cells(const SceneDesc* __
 template <int D>
 __global__ void k_clear_sgrid(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
-  clear_list_grid<D>(T);
+  clear_list_grid<D>(T, blockIdx.x, gridDim.x);
 }
 
 // before a bucketing pass that does not follow a rebuild in the same graph, in one launch:
@@ -351,13 +365,11 @@ __global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   const size_t st = (size_t)gridDim.x * blockDim.x;
   const size_t t0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  clear_list_grid<D>(T);
-  fill_zero((uint32_t*)sd.S.bcnt, (size_t)T.total_blocks);
-  fill_zero((uint32_t*)sd.S.frozen, 1);
+  clear_list_grid<D>(T, blockIdx.x, gridDim.x);
+  clear_bucket_counts(sd, blockIdx.x, gridDim.x);
   uint32_t* a = (uint32_t*)T.lv[0].sflag;
   uint32_t* v = sd.vflag;
   for (size_t i = t0; i < sd.w_flags; i += st) a[i] = v[i];
-  if (t0 == 0) atomicAdd(&sd.cnt[CNT_STEP], 1ull);
 }
 
 // ------------------------------------------------------------------------------ activation
@@ -598,15 +610,21 @@ __device__ __forceinline__ void mark_level_corners(const Topo& T, int l, const i
 }
 
 // adaptive_gimp.py:142-153: corners of exactly-ACTIVATED cells get ACTIVATED; also the
-// level-l corner maps of the level map
+// level-l corner maps of the level map.  The last `ngrid` workgroups instead zero the grid of
+// the previous substep's grid-phase lists (when no earlier launch of this rebuild did).
 template <int D>
-__global__ void k_node_A(const SceneDesc* __restrict__ Dsc) {
+__global__ void k_node_A(const SceneDesc* __restrict__ Dsc, int ngrid) {
   AGIMP_SCENE
+  const int nwg = gridDim.x - ngrid;
+  if ((int)blockIdx.x >= nwg) {
+    clear_list_grid<D>(T, blockIdx.x - nwg, ngrid);
+    return;
+  }
   const int* list = T.clist;
   const int* countp = T.ccount + 2;
   unsigned long long* cnt = sd.cnt;
   const int nlist = *countp;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += nwg * blockDim.x) {
   int l, I[3];
   uint32_t q;
   if (!decode_list_wave<D>(T, list, nlist, t, l, I, q)) continue;
@@ -703,7 +721,12 @@ __device__ __forceinline__ void mark_G_face(const Topo& T, unsigned long long* c
     for (int a = 0; a < D; ++a) P[a] = I1[a] >> (l - l0);
     int64_t qc = lidx<D>(T.lv[l0], P);
     if (qc < 0) { count(cnt, CNT_OOB); continue; }
-    T.lv[l0].cell[qc] |= GHOST;
+    // |= GHOST as a plain store: the cell lies below the neighbour's finest exactly-ACTIVATED
+    // ancestor (at lp), a leaf of the octree, so it holds no ACTIVATED bit (an activated
+    // descendant would have culled that ancestor; static octrees have no finer activated cell
+    // there by the choice of lp), no CULLING bit (set only on ancestors of activated cells) and
+    // no COVERED bit yet (set after the marks): it holds 0 or GHOST.  No load to wait for.
+    T.lv[l0].cell[qc] = GHOST;
     mark_cell_block<D>(T, l0, qc);
     const Corners<D> C(T, l0, P);
     uint32_t sel = 0;
@@ -2137,9 +2160,20 @@ __device__ __forceinline__ void g2p_gather(const LevelDesc& Ll, int pow2, const 
 
 // adaptive_gimp.py:252-275 (+ plasticity and the next substep's stress); reads slot
 // perm[pos] of the current buffer, writes slot pos of the next buffer (re-sorting for free).
+// The last `nclear` workgroups (when the next substep's rebuild follows in the same graph, so
+// nothing reads this octree any more) clear the octree's marks and the bucketing counters for
+// it; they start as G2P's own workgroups drain.
 template <int D>
-__global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, float dt_arg, int par) {
+__global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, float dt_arg, int par, int nclear) {
   AGIMP_SCENE
+  {
+    const int nwg = gridDim.x - nclear;
+    if ((int)blockIdx.x >= nwg) {
+      clear_octree<D>(T, blockIdx.x - nwg, nclear);
+      clear_bucket_counts(sd, blockIdx.x - nwg, nclear);
+      return;
+    }
+  }
   const PBuf& Pin = sd.P[par];
   const PBuf& Pout = sd.P[par ^ 1];
   const int* perm = sd.S.perm;
@@ -2370,7 +2404,8 @@ static inline dim3 grid2(size_t x, int nscenes) {
   return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(x, 1u << 20)), (unsigned)nscenes);
 }
 
-void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn, bool chained) {
+void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn, bool chained,
+                   bool precleared) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 rg = grid2(std::min<size_t>(blocks_for(m.max_total_blocks, 256), (size_t)m.fill_grid), m.nscenes);
@@ -2378,7 +2413,9 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
   const dim3 fg = grid2(std::min<size_t>(blocks_for(m.max_fill_words, 256), (size_t)m.fill_grid), m.nscenes);
   AGIMP_DIM(m.dim, {
     if (!chained) AGIMP_LAUNCH("k_clear_sgrid", k_clear_sgrid<DD>, lg, dim3(256), 0, s, d);
-    AGIMP_LAUNCH("k_clear_refine", k_clear_refine<DD>, rg, dim3(256), 0, s, d, chained ? 1 : 0);
+    // precleared (chained only): the previous substep's G2P launch cleared the octree and the
+    // bucketing counters; the grid of its lists is zeroed beside the node marks
+    if (!precleared) AGIMP_LAUNCH("k_clear_refine", k_clear_refine<DD>, rg, dim3(256), 0, s, d, chained ? 1 : 0);
     if (m.any_static) AGIMP_LAUNCH("k_activate_static", k_activate_static<DD>, pg, dim3(256), 0, s, d, par);
     if (m.any_dynamic) {
       AGIMP_LAUNCH("k_activate_dynamic", k_activate_dynamic<DD>, pg, dim3(256), 0, s, d, par);
@@ -2391,7 +2428,10 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
       }
     }
     AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 2);
-    AGIMP_LAUNCH("k_node_A", k_node_A<DD>, lg, dim3(256), 0, s, d);
+    {
+      const int ngrid = precleared ? (int)rg.x : 0;
+      AGIMP_LAUNCH("k_node_A", k_node_A<DD>, dim3(lg.x + ngrid, lg.y), dim3(256), 0, s, d, ngrid);
+    }
     AGIMP_LAUNCH("k_mark_TG", k_mark_TG<DD>, lg, dim3(256), 0, s, d);
     if (!chained) AGIMP_LAUNCH("k_copy_flags", k_copy_flags, fg, dim3(256), 0, s, d, 1);
     if (learn)
@@ -2446,9 +2486,10 @@ void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
   });
 }
 
-void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par) {
-  AGIMP_DIM(m.dim,
-            AGIMP_LAUNCH("k_g2p", k_g2p<DD>, grid2(blocks_for(m.max_n, 256), m.nscenes), dim3(256), 0, s, d, dt, par));
+void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par, bool clear_next) {
+  const int nclear = clear_next ? (int)std::min<size_t>(blocks_for(m.max_total_blocks, 256), (size_t)m.fill_grid) : 0;
+  const dim3 g = grid2(blocks_for(m.max_n, 256) + nclear, m.nscenes);
+  AGIMP_DIM(m.dim, AGIMP_LAUNCH("k_g2p", k_g2p<DD>, g, dim3(256), 0, s, d, dt, par, nclear));
 }
 
 void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n, int par) {

@@ -49,12 +49,17 @@ size_t cells_total(const Topo& T, int lmin, int lmax);
 // chained).  The rebuild's clear then also zeroes the previous grid-phase lists' grid, and the
 // bucketing pass saves the octree's grid-phase block flags, which an unchained rebuild saves
 // itself (k_copy_flags) and an unchained bucketing pass restores.
+// precleared (chained only): the previous substep's G2P launch ran with clear_next, so the
+// rebuild skips its clear launch and zeroes the previous lists' grid beside the node marks.
 void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn = nullptr,
-                   bool chained = false);
+                   bool chained = false, bool precleared = false);
 void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained = false);
 void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt);
-void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
+// clear_next: a chained rebuild with precleared follows in the same stream and nothing reads
+// this substep's octree before it; G2P's launch then also clears the octree's marks and the
+// bucketing counters.
+void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par, bool clear_next = false);
 void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n, int par);
 void launch_upload(hipStream_t s, const SceneDesc* d, int par, int dim, int n, const float* x, const float* v,
                    const float* F, const float* C, const float* m, const int* g, const float* Jp);
