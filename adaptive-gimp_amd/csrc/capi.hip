@@ -607,11 +607,13 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   T.total_blocks = gbase;
   uint8_t* cellbuf = nullptr;
   uint8_t* cornerbuf = nullptr;
+  uint8_t* splitbuf = nullptr;
   float4* gridbuf = nullptr;
   uint8_t* flagbuf = nullptr;
   int* slistbuf = nullptr;
   rc = rc ? rc : dalloc(h, &cellbuf, (cells + 3) & ~(size_t)3);
   rc = rc ? rc : dalloc(h, &cornerbuf, (cells + 3) & ~(size_t)3);
+  rc = rc ? rc : dalloc(h, &splitbuf, (cells + 3) & ~(size_t)3);
   rc = rc ? rc : dalloc(h, &gridbuf, cells);
   rc = rc ? rc : dalloc(h, &flagbuf, (flags + 3) & ~(size_t)3);
   rc = rc ? rc : dalloc(h, &slistbuf, flags);
@@ -626,6 +628,7 @@ int agimp_create(const agimp_config* cfg, agimp_handle** out) {
   for (int l = 0; l < T.L; ++l) {
     T.lv[l].cell = as_global(cellbuf + co);
     T.lv[l].corner = as_global(cornerbuf + co);
+    T.lv[l].split = as_global(splitbuf + co);
     T.lv[l].grid = as_global(gridbuf + co);
     T.lv[l].sflag = as_global(flagbuf + fo);
     T.lv[l].slist = as_global(slistbuf + fo);

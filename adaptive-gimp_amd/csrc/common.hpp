@@ -54,6 +54,9 @@ struct LevelDesc {
   // l are the cells of which its base cell + 1 is a corner, so one entry answers whether one
   // of them is exactly ACTIVATED (the level map's test).
   uint8_t AGIMP_GLOBAL* corner;
+  // [nblocks * 4^d] in the cells' layout: 1 if the culling pass split-activated the cell (its
+  // parent got CULLING); merged into the cell's ACTIVATED bit by the node-marks pass
+  uint8_t AGIMP_GLOBAL* split;
   float4 AGIMP_GLOBAL* grid;    // [nblocks * 4^d]
   uint8_t AGIMP_GLOBAL* sflag;  // [nblocks]   block must be processed by the grid phase
   int AGIMP_GLOBAL* slist;      // [nblocks]   compacted sflag list

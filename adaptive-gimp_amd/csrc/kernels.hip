@@ -233,13 +233,14 @@ __device__ __forceinline__ void clear_list_grid(const Topo& T, int wg, int nwg) 
   }
 }
 
-// The octree's marks, before a rebuild: cell masks, corner maps, node mask, block flags, the
-// compaction counters.  Only flagged blocks can hold marks: every cell or corner-map write
-// flags its leaf block (cflag), and every node mark lies in a finest-level block whose
+// The octree's marks, before a rebuild: cell masks, corner and split maps, node mask, block
+// flags, the compaction counters.  Only flagged blocks can hold marks: every cell, corner-map
+// or split-map write flags its leaf block (cflag), and every node mark lies in a finest-level
+// block whose
 // grid-phase flag (sflag) is set (the node marks set it, it is saved after refine and
 // restored, never cleared, by bucketing).  One thread per global block zeroes the block's
-// cells and corners if cflag is set and, for finest blocks with sflag set, its nodes; then both
-// flags.  A thread reads and clears only its own block's flags.
+// cells, corners and splits if cflag is set and, for finest blocks with sflag set, its nodes;
+// then both flags.  A thread reads and clears only its own block's flags.
 template <int D>
 __device__ __forceinline__ void clear_octree(const Topo& T, int wg, int nwg) {
   constexpr int WPB = Pow4<D>::v / 4;  // 32-bit words per leaf block of byte masks
@@ -247,6 +248,7 @@ __device__ __forceinline__ void clear_octree(const Topo& T, int wg, int nwg) {
   uint8_t AGIMP_GLOBAL* sflag = T.lv[0].sflag;
   uint32_t AGIMP_GLOBAL* cells = (uint32_t AGIMP_GLOBAL*)T.lv[0].cell;  // block g at word g * WPB
   uint32_t AGIMP_GLOBAL* corners = (uint32_t AGIMP_GLOBAL*)T.lv[0].corner;  // the same layout
+  uint32_t AGIMP_GLOBAL* splits = (uint32_t AGIMP_GLOBAL*)T.lv[0].split;    // the same layout
   uint32_t AGIMP_GLOBAL* nodes = (uint32_t AGIMP_GLOBAL*)T.node;
   const int st = nwg * blockDim.x;
   for (int g = wg * blockDim.x + threadIdx.x; g < nb; g += st) {
@@ -255,6 +257,8 @@ __device__ __forceinline__ void clear_octree(const Topo& T, int wg, int nwg) {
       for (int w = 0; w < WPB; ++w) cells[(size_t)g * WPB + w] = 0u;
 #pragma unroll
       for (int w = 0; w < WPB; ++w) corners[(size_t)g * WPB + w] = 0u;
+#pragma unroll
+      for (int w = 0; w < WPB; ++w) splits[(size_t)g * WPB + w] = 0u;
       T.cflag[g] = 0;
     }
     if (sflag[g]) {
@@ -510,9 +514,21 @@ __global__ void k_activate_static(const SceneDesc* __restrict__ Dsc, int par) {
 }
 
 // ------------------------------------------------------------------------------ culling
-// adaptive_gimp.py:98-109 for all l >= 1 at once (ACTIVATED bits do not change here)
+// adaptive_gimp.py:98-118 for all levels at once, in one pass.  Every cell of level l >= 1
+// holding ACTIVATED marks CULLING on its ancestors from its finest activated ancestor (level
+// l0) down to level l-1 (:98-109; ACTIVATED bits do not change here), and the 2^d children of
+// each of those ancestors are split-activated (:111-118): every CULLING cell is such an
+// ancestor, so this reaches the children of all of them.  The split goes to a map of its own
+// (LevelDesc::split) and k_node_A ORs ACTIVATED into those cells before anything compares a
+// cell with ACTIVATED again; so the ancestor searches here see only the activation's bits, as
+// in the reference, where the split follows all culling marks.  The culled cell itself keeps
+// A|C (it compares unequal to ACTIVATED from here on).  Children of one ancestor lie in one
+// leaf block (2P and 2P+1 share a block), flagged once.  A cull's children always lie in the
+// padded box (the finer boxes cover less than the coarser ones, and a culled cell has a child
+// in the box), so the OOB count of a child outside (per cull here, per culled cell in the
+// reference) is never reached.
 template <int D>
-__global__ void k_mark_hier(const SceneDesc* __restrict__ Dsc) {
+__global__ void k_cull(const SceneDesc* __restrict__ Dsc) {
   AGIMP_SCENE
   if (sd.static_mode) return;
   const int* list = T.clist;
@@ -539,38 +555,22 @@ __global__ void k_mark_hier(const SceneDesc* __restrict__ Dsc) {
 #pragma unroll
       for (int a = 0; a < D; ++a) P[a] = I[a] >> (l - lq);
       int64_t qq = lidx<D>(T.lv[lq], P);
-      if (qq >= 0) {
-        T.lv[lq].cell[qq] |= CULLING;
-        mark_cell_block<D>(T, lq, qq);
+      if (qq < 0) continue;
+      T.lv[lq].cell[qq] |= CULLING;
+      mark_cell_block<D>(T, lq, qq);
+      const LevelDesc Lc = T.lv[lq + 1];
+      int64_t qc = -1;
+#pragma unroll
+      for (int u = 0; u < (1 << D); ++u) {
+        int Jc[3];
+#pragma unroll
+        for (int a = 0; a < D; ++a) Jc[a] = 2 * P[a] + ((u >> (D - 1 - a)) & 1);
+        const int64_t qu = lidx<D>(Lc, Jc);
+        if (qu < 0) { count(cnt, CNT_OOB); continue; }
+        Lc.split[qu] = 1;
+        qc = qu;
       }
-    }
-  }
-}
-
-// adaptive_gimp.py:111-118 for all l at once: children of CULLING cells get ACTIVATED.
-// The culled cell itself keeps A|C (it compares unequal to ACTIVATED from here on).
-template <int D>
-__global__ void k_split(const SceneDesc* __restrict__ Dsc) {
-  AGIMP_SCENE
-  if (sd.static_mode) return;
-  const int* list = T.clist;
-  const int* countp = T.ccount + 1;
-  unsigned long long* cnt = sd.cnt;
-  const int nlist = *countp;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nlist * Pow4<D>::v; t += gridDim.x * blockDim.x) {
-    int l, I[3];
-    uint32_t q;
-    if (!decode_list_wave<D>(T, list, nlist, t, l, I, q) || l > T.L - 2) continue;
-    if (!(T.lv[l].cell[q] & CULLING)) continue;
-#pragma unroll
-    for (int u = 0; u < (1 << D); ++u) {
-      int Jc[3];
-#pragma unroll
-      for (int a = 0; a < D; ++a) Jc[a] = 2 * I[a] + ((u >> (D - 1 - a)) & 1);
-      int64_t qq = lidx<D>(T.lv[l + 1], Jc);
-      if (qq < 0) { count(cnt, CNT_OOB); continue; }
-      T.lv[l + 1].cell[qq] |= ACTIVATED;
-      mark_cell_block<D>(T, l + 1, qq);
+      if (qc >= 0) mark_cell_block<D>(T, lq + 1, qc);
     }
   }
 }
@@ -628,7 +628,12 @@ __global__ void k_node_A(const SceneDesc* __restrict__ Dsc, int ngrid) {
   int l, I[3];
   uint32_t q;
   if (!decode_list_wave<D>(T, list, nlist, t, l, I, q)) continue;
-  if (T.lv[l].cell[q] != ACTIVATED) continue;
+  uint8_t c = T.lv[l].cell[q];
+  if (T.lv[l].split[q]) {  // split-activated by the culling pass: the cell gets ACTIVATED now
+    c |= ACTIVATED;
+    T.lv[l].cell[q] = c;
+  }
+  if (c != ACTIVATED) continue;
   const Corners<D> C(T, l, I);
   uint32_t sel = 0;
 #pragma unroll
@@ -2422,9 +2427,7 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
       if (learn) launch_unit_activate(s, d, m.dim, (int)m.max_n, par, *learn);
       if (m.maxL > 1) {
         AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 0);
-        AGIMP_LAUNCH("k_mark_hier", k_mark_hier<DD>, lg, dim3(256), 0, s, d);
-        AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 1);
-        AGIMP_LAUNCH("k_split", k_split<DD>, lg, dim3(256), 0, s, d);
+        AGIMP_LAUNCH("k_cull", k_cull<DD>, lg, dim3(256), 0, s, d);
       }
     }
     AGIMP_LAUNCH("k_compact_cells", k_compact_cells, cg, dim3(COMPACT_T), 0, s, d, 2);

@@ -118,7 +118,7 @@ def cpu_baseline(workloads, seconds_target=12.0, seconds_1t=8.0):
 # kernels of each substep phase (launch names of the library's per-kernel timer)
 PHASE_KERNELS = {
     "refine": ("k_clear_sgrid", "k_clear_refine", "k_activate_static", "k_activate_dynamic", "k_compact_cells",
-               "k_mark_hier", "k_split", "k_node_A", "k_mark_TG", "k_copy_flags"),
+               "k_cull", "k_node_A", "k_mark_TG", "k_copy_flags"),
     "bucket": ("k_bucket_prep", "k_levelmap", "k_scan_reduce", "k_scan_final", "k_scatter_blocks"),
     "p2g": ("k_p2g",),
     "grid": ("k_restrict", "k_gridop", "k_prolong"),
