@@ -748,16 +748,17 @@ __device__ __forceinline__ void mark_G_face(const Topo& T, unsigned long long* c
   }
 }
 
+// (workgroup wg of the nwg that take part)
 template <int D>
-__global__ void k_mark_TG(const SceneDesc* __restrict__ Dsc) {
-  AGIMP_SCENE
+__device__ __forceinline__ void mark_TG_wg(const SceneDesc& sd, int wg, int nwg) {
+  const Topo& T = sd.T;
   const int* list = T.clist;
   unsigned long long* cnt = sd.cnt;
   const int nlist = T.ccount[2];
   const int N = nlist * Pow4<D>::v;
   const int lane = __lane_id();
   // the loop is uniform over each workgroup, so every lane takes part in the shuffles
-  for (int base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
+  for (int base = wg * blockDim.x; base < N; base += nwg * blockDim.x) {
     const int t = base + threadIdx.x;
     int l = 0, I[3] = {0, 0, 0};
     uint32_t q;
@@ -849,6 +850,11 @@ __global__ void k_mark_TG(const SceneDesc* __restrict__ Dsc) {
       mark_G_face<D>(T, cnt, I1, ls, lpj);
     }
   }
+}
+
+template <int D>
+__global__ void k_mark_TG(const SceneDesc* __restrict__ Dsc) {
+  mark_TG_wg<D>(Dsc[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 // Coverage of level-l cells by finer activated cells, level by level from the finest up (one
@@ -1039,28 +1045,20 @@ __device__ __forceinline__ int lm_finish(const SceneDesc& sd, const PBuf& P, int
 // then one global atomic per (workgroup, block) reserves the workgroup's range: consecutive
 // wavefronts mostly fill the same block, and one returning global atomic per run made them
 // queue on its counter.
-// save_flags (chained after a rebuild): also save the octree's grid-phase block flags (the
-// chunk pass adds to them next), the part of the bucketing preparation that must follow the
-// rebuild's node marks.
+// (workgroup wg of the launch's level-map part)
 template <int D, int NL>
-__global__ void __launch_bounds__(LM_T) k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int save_flags) {
-  AGIMP_SCENE
+__device__ __forceinline__ void levelmap_wg(const SceneDesc& sd, int par, int wg) {
+  const Topo& T = sd.T;
   const PBuf& P = sd.P[par];
   const int n = sd.n;
   int* gb = sd.S.gb;
   int* rk = sd.S.rk;
   int* bcnt = sd.S.bcnt;
   int* frozen = sd.S.frozen;
-  if (save_flags) {
-    const uint32_t* a = (const uint32_t*)T.lv[0].sflag;
-    uint32_t* v = sd.vflag;
-    const size_t st = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < sd.w_flags; i += st) v[i] = a[i];
-  }
   // the grid-phase list counters (the prep pass read them; the block-list compaction in the
   // P2G launch appends from zero)
-  if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
-  const int k0 = blockIdx.x * (LM_T * LM_P) + threadIdx.x;
+  if (wg == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
+  const int k0 = wg * (LM_T * LM_P) + threadIdx.x;
   if (k0 - (int)threadIdx.x >= n) return;  // whole workgroup past this scene
   __shared__ int tkey[LM_TABLE], tcnt[LM_TABLE], tbase[LM_TABLE];
   for (int i = threadIdx.x; i < LM_TABLE; i += LM_T) {
@@ -1129,6 +1127,22 @@ __global__ void __launch_bounds__(LM_T) k_levelmap(const SceneDesc* __restrict__
   }
 }
 
+template <int D, int NL>
+__global__ void __launch_bounds__(LM_T) k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
+  levelmap_wg<D, NL>(Dsc[blockIdx.y], par, blockIdx.x);
+}
+
+// In a substep captured whole (the rebuild's last pass and the bucketing pass's first in one
+// launch): the first ntg workgroups make the T and ghost marks, the others the level map.
+// Independent: the level map reads the corner maps and cells that the node-marks pass left
+// (ghost marks write GHOST only into cells that are not exactly ACTIVATED and into no corner
+// map), and neither reads what the other writes; both are latency-bound and overlap.
+template <int D, int NL>
+__global__ void __launch_bounds__(LM_T) k_marks_levelmap(const SceneDesc* __restrict__ Dsc, int par, int ntg) {
+  if ((int)blockIdx.x < ntg) mark_TG_wg<D>(Dsc[blockIdx.y], blockIdx.x, ntg);
+  else levelmap_wg<D, NL>(Dsc[blockIdx.y], par, blockIdx.x - ntg);
+}
+
 // exclusive scan of (count, chunks) over all global blocks: 256 threads x 8 items per tile
 constexpr int SCAN_T = 256, SCAN_I = 8, SCAN_TILE = SCAN_T * SCAN_I;
 
@@ -1182,8 +1196,17 @@ __device__ __forceinline__ void load_cc(const int* bcnt, int base, int n, int CH
   for (int i = 0; i < SCAN_I; ++i) v[i] = make_int2(c[i], (c[i] + CH - 1) / CH);
 }
 
-__global__ void k_scan_reduce(const SceneDesc* __restrict__ Dsc) {
+// save_flags (after a rebuild in the same graph): also save the octree's grid-phase block
+// flags (the scatter adds to them next), the part of the bucketing preparation that must
+// follow the rebuild's node, T and ghost marks.
+__global__ void k_scan_reduce(const SceneDesc* __restrict__ Dsc, int save_flags) {
   AGIMP_SCENE
+  if (save_flags) {
+    const uint32_t* a = (const uint32_t*)T.lv[0].sflag;
+    uint32_t* v = sd.vflag;
+    const size_t st = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < sd.w_flags; i += st) v[i] = a[i];
+  }
   const int* bcnt = sd.S.bcnt;
   const int n = T.total_blocks, CH = sd.CH;
   int2* tile_sum = sd.S.tiles;
@@ -2435,11 +2458,21 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
       const int ngrid = precleared ? (int)rg.x : 0;
       AGIMP_LAUNCH("k_node_A", k_node_A<DD>, dim3(lg.x + ngrid, lg.y), dim3(256), 0, s, d, ngrid);
     }
-    AGIMP_LAUNCH("k_mark_TG", k_mark_TG<DD>, lg, dim3(256), 0, s, d);
+    // chained: the T and ghost marks run in the bucketing pass's first launch, beside the level map
+    if (!chained) AGIMP_LAUNCH("k_mark_TG", k_mark_TG<DD>, lg, dim3(256), 0, s, d);
     if (!chained) AGIMP_LAUNCH("k_copy_flags", k_copy_flags, fg, dim3(256), 0, s, d, 1);
     if (learn)
       for (int l = m.maxL - 2; l >= 0; --l) AGIMP_LAUNCH("k_cover", k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);
   });
+}
+
+template <int D, int NL>
+static void launch_levelmap(hipStream_t s, const SceneDesc* d, int par, bool chained, dim3 lg, dim3 lmg) {
+  if (chained)
+    AGIMP_LAUNCH("k_marks_levelmap", (k_marks_levelmap<D, NL>), dim3(lg.x + lmg.x, lmg.y), dim3(LM_T), 0, s, d, par,
+                 (int)lg.x);
+  else
+    AGIMP_LAUNCH("k_levelmap", (k_levelmap<D, NL>), lmg, dim3(LM_T), 0, s, d, par);
 }
 
 void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained) {
@@ -2453,17 +2486,18 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bo
     if (!chained) AGIMP_LAUNCH("k_bucket_prep", k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
     {
       const dim3 lmg = grid2(blocks_for(m.max_n, LM_T * LM_P), m.nscenes);
-      const int sv = chained ? 1 : 0;
-      switch (m.maxL) {  // the level loop unrolled for the most levels of the launch's scenes
-        case 1: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 1>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
-        case 2: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 2>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
-        case 3: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 3>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
-        case 4: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 4>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
-        case 5: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, 5>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
-        default: AGIMP_LAUNCH("k_levelmap", (k_levelmap<DD, MAXL>), lmg, dim3(LM_T), 0, s, d, par, sv); break;
+      // the level loop unrolled for the most levels of the launch's scenes; chained (after a
+      // rebuild in the same graph): the rebuild's T and ghost marks in the same launch
+      switch (m.maxL) {
+        case 1: launch_levelmap<DD, 1>(s, d, par, chained, lg, lmg); break;
+        case 2: launch_levelmap<DD, 2>(s, d, par, chained, lg, lmg); break;
+        case 3: launch_levelmap<DD, 3>(s, d, par, chained, lg, lmg); break;
+        case 4: launch_levelmap<DD, 4>(s, d, par, chained, lg, lmg); break;
+        case 5: launch_levelmap<DD, 5>(s, d, par, chained, lg, lmg); break;
+        default: launch_levelmap<DD, MAXL>(s, d, par, chained, lg, lmg); break;
       }
     }
-    AGIMP_LAUNCH("k_scan_reduce", k_scan_reduce, tg, dim3(SCAN_T), 0, s, d);
+    AGIMP_LAUNCH("k_scan_reduce", k_scan_reduce, tg, dim3(SCAN_T), 0, s, d, chained ? 1 : 0);
     AGIMP_LAUNCH("k_scan_final", k_scan_final, tg, dim3(SCAN_T), 0, s, d);
     AGIMP_LAUNCH("k_scatter_blocks", k_scatter_blocks<DD>, sbg, dim3(256), 0, s, d);
   });
