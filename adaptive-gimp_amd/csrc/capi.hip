@@ -80,6 +80,8 @@ struct agimp_handle {
   int* snap_id = nullptr;
   unsigned long long* snap_cnt = nullptr;
   Captured fixed[2];          // substep without the rebuild (bucketing + physics), per buffer index
+  Captured fixed_chain[2];    // a whole window of such substeps in one graph, per starting buffer
+  int fixed_chain_len = 0;    // the window length they were captured for
   int64_t lstats[AGIMP_NLSTATS] = {};
 };
 
@@ -395,6 +397,27 @@ hipError_t capture_fixed(hipStream_t s, const SceneDesc* dd, const Dims& m, int 
   return hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
 }
 
+// n substeps without the rebuild in one graph, starting from buffer `par` (a window of reuse of
+// one predicted octree: the host checks the refinement criterion only at the window's end, so
+// the window's substeps need no host launch each)
+hipError_t capture_fixed_chain(hipStream_t s, const SceneDesc* dd, const Dims& m, int par, int n, Captured& cap) {
+  cap.release();
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return e;
+  for (int i = 0; i < n; ++i) {
+    const int p = par ^ (i & 1);
+    launch_bucket(s, dd, m, p);
+    launch_p2g(s, dd, m, 0.f, p);
+    launch_grid(s, dd, m, 0.f);
+    launch_g2p(s, dd, m, 0.f, p);
+  }
+  hipError_t le = hipGetLastError();
+  e = hipStreamEndCapture(s, &cap.graph);
+  if (le != hipSuccess) return le;
+  if (e != hipSuccess) return e;
+  return hipGraphInstantiate(&cap.exec, cap.graph, nullptr, nullptr, 0);
+}
+
 // dt of graph replays lives in the device descriptor
 int set_graph_dt(agimp_handle* h, float dt) {
   if (dt == h->graph_dt) return 0;
@@ -688,6 +711,8 @@ int agimp_destroy(agimp_handle* h) {
   h->chain[1].release();
   h->fixed[0].release();
   h->fixed[1].release();
+  h->fixed_chain[0].release();
+  h->fixed_chain[1].release();
   for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
   for (auto& m : h->kmarks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
   if (h->ktimer.a) (void)hipEventDestroy(h->ktimer.a);
@@ -1095,6 +1120,18 @@ int agimp_learned_advance(agimp_handle* h, float dt, int32_t n) {
       return fail(h, AGIMP_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
     }
   }
+  if (h->window > 1 && h->fixed_chain_len != h->window) {  // whole windows from one graph each
+    for (int p = 0; p < 2; ++p) {
+      hipError_t e = capture_fixed_chain(h->stream, h->dd, h->dims, p, h->window, h->fixed_chain[p]);
+      if (e != hipSuccess) {
+        h->fixed_chain[0].release();
+        h->fixed_chain[1].release();
+        h->fixed_chain_len = 0;
+        return fail(h, AGIMP_EHIP, std::string("graph capture: ") + hipGetErrorString(e));
+      }
+    }
+    h->fixed_chain_len = h->window;
+  }
   const unsigned long long win_init[2] = {~0ull, 0ull};  // CNT_FIRSTVIOL, CNT_STEP
   int t = 0;
   while (t < n) {
@@ -1106,9 +1143,14 @@ int agimp_learned_advance(agimp_handle* h, float dt, int32_t n) {
     rc = snapshot(h, par0, true);
     if (rc) return rc;
     HIPCHK(h, hipMemcpyAsync(h->hd.cnt + CNT_FIRSTVIOL, win_init, sizeof(win_init), hipMemcpyHostToDevice, h->stream));
-    for (int i = 0; i < w; ++i) {
-      HIPCHK(h, hipGraphLaunch(h->fixed[h->host_parity].exec, h->stream));
-      h->host_parity ^= 1;
+    if (w > 1 && w == h->fixed_chain_len) {
+      HIPCHK(h, hipGraphLaunch(h->fixed_chain[h->host_parity].exec, h->stream));
+      h->host_parity ^= w & 1;
+    } else {
+      for (int i = 0; i < w; ++i) {
+        HIPCHK(h, hipGraphLaunch(h->fixed[h->host_parity].exec, h->stream));
+        h->host_parity ^= 1;
+      }
     }
     unsigned long long st[3], v0 = 0;  // CNT_VIOL, CNT_FIRSTVIOL, CNT_STEP; CNT_VIOL at the window start
     HIPCHK(h, hipMemcpyAsync(st, h->hd.cnt + CNT_VIOL, sizeof(st), hipMemcpyDeviceToHost, h->stream));
