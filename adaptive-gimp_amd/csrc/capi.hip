@@ -344,7 +344,7 @@ hipError_t capture_chain(hipStream_t s, const SceneDesc* dd, const Dims& m, int 
     launch_bucket(s, dd, m, p, true);
     launch_p2g(s, dd, m, 0.f, p);
     launch_grid(s, dd, m, 0.f);
-    launch_g2p(s, dd, m, 0.f, p, i + 1 < SUBSTEP_CHAIN);
+    launch_g2p(s, dd, m, 0.f, p, i + 1 < SUBSTEP_CHAIN ? G2P_NEXT_REBUILD : G2P_NEXT_NONE);
   }
   hipError_t le = hipGetLastError();
   e = hipStreamEndCapture(s, &cap.graph);
@@ -404,12 +404,13 @@ hipError_t capture_fixed_chain(hipStream_t s, const SceneDesc* dd, const Dims& m
   cap.release();
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
+  // each G2P launch but the last prepares the next substep's bucketing pass
   for (int i = 0; i < n; ++i) {
     const int p = par ^ (i & 1);
-    launch_bucket(s, dd, m, p);
+    launch_bucket(s, dd, m, p, false, i > 0);
     launch_p2g(s, dd, m, 0.f, p);
     launch_grid(s, dd, m, 0.f);
-    launch_g2p(s, dd, m, 0.f, p);
+    launch_g2p(s, dd, m, 0.f, p, i + 1 < n ? G2P_NEXT_FIXED : G2P_NEXT_NONE);
   }
   hipError_t le = hipGetLastError();
   e = hipStreamEndCapture(s, &cap.graph);

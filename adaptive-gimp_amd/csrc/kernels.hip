@@ -1055,9 +1055,6 @@ __device__ __forceinline__ void levelmap_wg(const SceneDesc& sd, int par, int wg
   int* rk = sd.S.rk;
   int* bcnt = sd.S.bcnt;
   int* frozen = sd.S.frozen;
-  // the grid-phase list counters (the prep pass read them; the block-list compaction in the
-  // P2G launch appends from zero)
-  if (wg == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
   const int k0 = wg * (LM_T * LM_P) + threadIdx.x;
   if (k0 - (int)threadIdx.x >= n) return;  // whole workgroup past this scene
   __shared__ int tkey[LM_TABLE], tcnt[LM_TABLE], tbase[LM_TABLE];
@@ -1127,9 +1124,13 @@ __device__ __forceinline__ void levelmap_wg(const SceneDesc& sd, int par, int wg
   }
 }
 
+// The last `ngrid` workgroups instead zero the grid of the previous substep's grid-phase lists
+// (when the previous G2P launch did the rest of the bucketing preparation).
 template <int D, int NL>
-__global__ void __launch_bounds__(LM_T) k_levelmap(const SceneDesc* __restrict__ Dsc, int par) {
-  levelmap_wg<D, NL>(Dsc[blockIdx.y], par, blockIdx.x);
+__global__ void __launch_bounds__(LM_T) k_levelmap(const SceneDesc* __restrict__ Dsc, int par, int ngrid) {
+  const int nwg = gridDim.x - ngrid;
+  if ((int)blockIdx.x >= nwg) clear_list_grid<D>(Dsc[blockIdx.y].T, blockIdx.x - nwg, ngrid);
+  else levelmap_wg<D, NL>(Dsc[blockIdx.y], par, blockIdx.x);
 }
 
 // In a substep captured whole (the rebuild's last pass and the bucketing pass's first in one
@@ -1234,6 +1235,9 @@ __global__ void k_scan_final(const SceneDesc* __restrict__ Dsc) {
   int* boff = sd.S.boff;
   int* choff = sd.S.choff;
   const int ntiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  // the grid-phase list counters: the block-list compaction in the P2G launch appends from
+  // zero (every clear of the previous lists' grid has run by now)
+  if (blockIdx.x == 0 && threadIdx.x < MAXL) T.scount[threadIdx.x] = 0;
   if ((int)blockIdx.x >= ntiles) return;
   __shared__ int2 sh[SCAN_T];
   // the preceding tiles' totals, 4 loads in flight per thread and round
@@ -2188,17 +2192,29 @@ __device__ __forceinline__ void g2p_gather(const LevelDesc& Ll, int pow2, const 
 
 // adaptive_gimp.py:252-275 (+ plasticity and the next substep's stress); reads slot
 // perm[pos] of the current buffer, writes slot pos of the next buffer (re-sorting for free).
-// The last `nclear` workgroups (when the next substep's rebuild follows in the same graph, so
-// nothing reads this octree any more) clear the octree's marks and the bucketing counters for
-// it; they start as G2P's own workgroups drain.
+// The last `nclear` workgroups (G2P_NEXT_REBUILD: the next substep's rebuild follows in the same
+// graph, so nothing reads this octree any more) clear the octree's marks and the bucketing
+// counters for it; they start as G2P's own workgroups drain.
+// G2P_NEXT_FIXED: the next substep, in the same graph, reuses this octree; the last `nclear`
+// workgroups then do its bucketing preparation instead (counters, and the octree's grid-phase
+// block flags restored from the saved copy).
 template <int D>
-__global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, float dt_arg, int par, int nclear) {
+__global__ void __launch_bounds__(256) k_g2p(const SceneDesc* __restrict__ Dsc, float dt_arg, int par, int nclear,
+                                             int next) {
   AGIMP_SCENE
   {
     const int nwg = gridDim.x - nclear;
     if ((int)blockIdx.x >= nwg) {
-      clear_octree<D>(T, blockIdx.x - nwg, nclear);
-      clear_bucket_counts(sd, blockIdx.x - nwg, nclear);
+      const int wg = blockIdx.x - nwg;
+      if (next == G2P_NEXT_REBUILD) {
+        clear_octree<D>(T, wg, nclear);
+      } else {
+        uint32_t* a = (uint32_t*)T.lv[0].sflag;
+        const uint32_t* v = sd.vflag;
+        const size_t st = (size_t)nclear * blockDim.x;
+        for (size_t i = wg * (size_t)blockDim.x + threadIdx.x; i < sd.w_flags; i += st) a[i] = v[i];
+      }
+      clear_bucket_counts(sd, wg, nclear);
       return;
     }
   }
@@ -2467,15 +2483,15 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
 }
 
 template <int D, int NL>
-static void launch_levelmap(hipStream_t s, const SceneDesc* d, int par, bool chained, dim3 lg, dim3 lmg) {
+static void launch_levelmap(hipStream_t s, const SceneDesc* d, int par, bool chained, int ngrid, dim3 lg, dim3 lmg) {
   if (chained)
     AGIMP_LAUNCH("k_marks_levelmap", (k_marks_levelmap<D, NL>), dim3(lg.x + lmg.x, lmg.y), dim3(LM_T), 0, s, d, par,
                  (int)lg.x);
   else
-    AGIMP_LAUNCH("k_levelmap", (k_levelmap<D, NL>), lmg, dim3(LM_T), 0, s, d, par);
+    AGIMP_LAUNCH("k_levelmap", (k_levelmap<D, NL>), dim3(lmg.x + ngrid, lmg.y), dim3(LM_T), 0, s, d, par, ngrid);
 }
 
-void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained) {
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained, bool prepped) {
   const dim3 pg = grid2(blocks_for(m.max_n, 256), m.nscenes);
   const dim3 cg = grid2(blocks_for((m.max_total_blocks + 3) / 4, COMPACT_T), m.nscenes);
   const dim3 lg = grid2(m.list_grid, m.nscenes);
@@ -2483,18 +2499,21 @@ void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bo
   const dim3 tg = grid2(blocks_for(m.max_total_blocks, SCAN_TILE), m.nscenes);
   const dim3 sbg = grid2(std::max(blocks_for(m.max_n, 256), blocks_for(m.max_total_blocks, 256)), m.nscenes);
   AGIMP_DIM(m.dim, {
-    if (!chained) AGIMP_LAUNCH("k_bucket_prep", k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
+    // prepped: the previous G2P launch did the preparation but the grid clear, which runs
+    // beside the level map
+    if (!chained && !prepped) AGIMP_LAUNCH("k_bucket_prep", k_bucket_prep<DD>, fg, dim3(256), 0, s, d);
+    const int ngrid = prepped ? (int)fg.x : 0;
     {
       const dim3 lmg = grid2(blocks_for(m.max_n, LM_T * LM_P), m.nscenes);
       // the level loop unrolled for the most levels of the launch's scenes; chained (after a
       // rebuild in the same graph): the rebuild's T and ghost marks in the same launch
       switch (m.maxL) {
-        case 1: launch_levelmap<DD, 1>(s, d, par, chained, lg, lmg); break;
-        case 2: launch_levelmap<DD, 2>(s, d, par, chained, lg, lmg); break;
-        case 3: launch_levelmap<DD, 3>(s, d, par, chained, lg, lmg); break;
-        case 4: launch_levelmap<DD, 4>(s, d, par, chained, lg, lmg); break;
-        case 5: launch_levelmap<DD, 5>(s, d, par, chained, lg, lmg); break;
-        default: launch_levelmap<DD, MAXL>(s, d, par, chained, lg, lmg); break;
+        case 1: launch_levelmap<DD, 1>(s, d, par, chained, ngrid, lg, lmg); break;
+        case 2: launch_levelmap<DD, 2>(s, d, par, chained, ngrid, lg, lmg); break;
+        case 3: launch_levelmap<DD, 3>(s, d, par, chained, ngrid, lg, lmg); break;
+        case 4: launch_levelmap<DD, 4>(s, d, par, chained, ngrid, lg, lmg); break;
+        case 5: launch_levelmap<DD, 5>(s, d, par, chained, ngrid, lg, lmg); break;
+        default: launch_levelmap<DD, MAXL>(s, d, par, chained, ngrid, lg, lmg); break;
       }
     }
     AGIMP_LAUNCH("k_scan_reduce", k_scan_reduce, tg, dim3(SCAN_T), 0, s, d, chained ? 1 : 0);
@@ -2523,10 +2542,13 @@ void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt) {
   });
 }
 
-void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par, bool clear_next) {
-  const int nclear = clear_next ? (int)std::min<size_t>(blocks_for(m.max_total_blocks, 256), (size_t)m.fill_grid) : 0;
+void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par, int next) {
+  const int nclear = next != G2P_NEXT_NONE
+                         ? (int)std::min<size_t>(blocks_for(next == G2P_NEXT_REBUILD ? m.max_total_blocks : m.max_fill_words, 256),
+                                                 (size_t)m.fill_grid)
+                         : 0;
   const dim3 g = grid2(blocks_for(m.max_n, 256) + nclear, m.nscenes);
-  AGIMP_DIM(m.dim, AGIMP_LAUNCH("k_g2p", k_g2p<DD>, g, dim3(256), 0, s, d, dt, par, nclear));
+  AGIMP_DIM(m.dim, AGIMP_LAUNCH("k_g2p", k_g2p<DD>, g, dim3(256), 0, s, d, dt, par, nclear, next));
 }
 
 void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n, int par) {

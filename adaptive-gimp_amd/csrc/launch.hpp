@@ -53,13 +53,17 @@ size_t cells_total(const Topo& T, int lmin, int lmax);
 // rebuild skips its clear launch and zeroes the previous lists' grid beside the node marks.
 void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, const LearnDesc* learn = nullptr,
                    bool chained = false, bool precleared = false);
-void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained = false);
+// prepped (unchained only): the previous substep's G2P launch ran with G2P_NEXT_FIXED, so the
+// preparation launch is skipped and the previous lists' grid is zeroed beside the level map.
+void launch_bucket(hipStream_t s, const SceneDesc* d, const Dims& m, int par, bool chained = false,
+                   bool prepped = false);
 void launch_p2g(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par);
 void launch_grid(hipStream_t s, const SceneDesc* d, const Dims& m, float dt);
-// clear_next: a chained rebuild with precleared follows in the same stream and nothing reads
-// this substep's octree before it; G2P's launch then also clears the octree's marks and the
-// bucketing counters.
-void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par, bool clear_next = false);
+// next: what follows in the same stream: G2P_NEXT_REBUILD, a chained rebuild with precleared;
+// G2P_NEXT_FIXED, a bucketing pass with prepped on this octree.  G2P's launch then does that
+// pass's preparation.
+enum G2pNext : int { G2P_NEXT_NONE = 0, G2P_NEXT_REBUILD = 1, G2P_NEXT_FIXED = 2 };
+void launch_g2p(hipStream_t s, const SceneDesc* d, const Dims& m, float dt, int par, int next = G2P_NEXT_NONE);
 void launch_stress(hipStream_t s, const SceneDesc* d, int dim, int n, int par);
 void launch_upload(hipStream_t s, const SceneDesc* d, int par, int dim, int n, const float* x, const float* v,
                    const float* F, const float* C, const float* m, const int* g, const float* Jp);
