@@ -75,10 +75,9 @@ struct agimp_handle {
   int window = 20;
   LearnDesc U{};
   int* cls_dev = nullptr;     // [nu] per-candidate classes (labels or predictions)
-  float* snap_f = nullptr;    // snapshot of the current particle buffer at a window start
-  int* snap_g = nullptr;
-  int* snap_id = nullptr;
-  unsigned long long* snap_cnt = nullptr;
+  unsigned long long* snap_cnt = nullptr;  // the counters at a window start (the particles: hd.spare)
+  unsigned long long* win_out = nullptr;   // [4] pinned host memory mapped for the device: a window's check
+  unsigned long long* win_out_dev = nullptr;
   Captured fixed[2];          // substep without the rebuild (bucketing + physics), per buffer index
   Captured fixed_chain[2];    // a whole window of such substeps in one graph, per starting buffer
   int fixed_chain_len = 0;    // the window length they were captured for
@@ -400,17 +399,20 @@ hipError_t capture_fixed(hipStream_t s, const SceneDesc* dd, const Dims& m, int 
 // n substeps without the rebuild in one graph, starting from buffer `par` (a window of reuse of
 // one predicted octree: the host checks the refinement criterion only at the window's end, so
 // the window's substeps need no host launch each)
-hipError_t capture_fixed_chain(hipStream_t s, const SceneDesc* dd, const Dims& m, int par, int n, Captured& cap) {
+hipError_t capture_fixed_chain(hipStream_t s, SceneDesc* dd, const Dims& m, int par, int n, Captured& cap) {
   cap.release();
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e != hipSuccess) return e;
-  // each G2P launch but the last prepares the next substep's bucketing pass
+  // each G2P launch but the last prepares the next substep's bucketing pass; after the first
+  // substep the window start's buffer (par, read by it) is swapped with the spare one, which
+  // the second substep then overwrites instead
   for (int i = 0; i < n; ++i) {
     const int p = par ^ (i & 1);
     launch_bucket(s, dd, m, p, false, i > 0);
     launch_p2g(s, dd, m, 0.f, p);
     launch_grid(s, dd, m, 0.f);
     launch_g2p(s, dd, m, 0.f, p, i + 1 < n ? G2P_NEXT_FIXED : G2P_NEXT_NONE);
+    if (i == 0) launch_swap_spare(s, dd, par);
   }
   hipError_t le = hipGetLastError();
   e = hipStreamEndCapture(s, &cap.graph);
@@ -471,23 +473,6 @@ size_t particle_floats(const agimp_handle* h) {
   return (size_t)NF * (size_t)h->hd.P[0].stride;
 }
 
-int snapshot(agimp_handle* h, int par, bool save) {
-  const PBuf& P = h->hd.P[par];
-  const size_t nf = particle_floats(h) * sizeof(float), ni = (size_t)P.stride * sizeof(int);
-  const hipMemcpyKind k = hipMemcpyDeviceToDevice;
-  if (save) {
-    HIPCHK(h, hipMemcpyAsync(h->snap_f, P.f, nf, k, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->snap_g, P.g, ni, k, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->snap_id, P.id, ni, k, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->snap_cnt, h->hd.cnt, NCOUNTERS * 8, k, h->stream));
-  } else {
-    HIPCHK(h, hipMemcpyAsync(P.f, h->snap_f, nf, k, h->stream));
-    HIPCHK(h, hipMemcpyAsync(P.g, h->snap_g, ni, k, h->stream));
-    HIPCHK(h, hipMemcpyAsync(P.id, h->snap_id, ni, k, h->stream));
-    HIPCHK(h, hipMemcpyAsync(h->hd.cnt, h->snap_cnt, NCOUNTERS * 8, k, h->stream));
-  }
-  return 0;
-}
 
 // Staging of agimp_get_particles: x, v [n][d], F, C [n][d][d], Jp [n] float32, fl, g [n] int32
 // in caller order, written by k_download, then copied to the host.
@@ -714,6 +699,7 @@ int agimp_destroy(agimp_handle* h) {
   h->fixed[1].release();
   h->fixed_chain[0].release();
   h->fixed_chain[1].release();
+  if (h->win_out) (void)hipHostFree(h->win_out);
   for (auto& m : h->marks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
   for (auto& m : h->kmarks) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
   if (h->ktimer.a) (void)hipEventDestroy(h->ktimer.a);
@@ -1007,11 +993,16 @@ int agimp_learned_enable(agimp_handle* h, int32_t window, float tau) {
     float* params = nullptr;
     rc = rc ? rc : dalloc(h, &params, (size_t)AGIMP_MLP_NPARAMS);
     rc = rc ? rc : dalloc(h, &h->cls_dev, nu);
-    rc = rc ? rc : dalloc(h, &h->snap_f, particle_floats(h));
-    rc = rc ? rc : dalloc(h, &h->snap_g, (size_t)h->hd.P[0].stride);
-    rc = rc ? rc : dalloc(h, &h->snap_id, (size_t)h->hd.P[0].stride);
+    PBuf& sp = h->hd.spare;
+    sp.stride = h->hd.P[0].stride;
+    rc = rc ? rc : dalloc(h, &sp.f, particle_floats(h));
+    rc = rc ? rc : dalloc(h, &sp.g, (size_t)sp.stride);
+    rc = rc ? rc : dalloc(h, &sp.id, (size_t)sp.stride);
     rc = rc ? rc : dalloc(h, &h->snap_cnt, NCOUNTERS);
     if (rc) return rc;
+    if (hipHostMalloc((void**)&h->win_out, 4 * sizeof(unsigned long long), hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&h->win_out_dev, h->win_out, 0) != hipSuccess)
+      return fail(h, AGIMP_ENOMEM, "hipHostMalloc failed for the window check's readback");
     HIPCHK(h, hipMemsetAsync(U.upred, 0xff, nu4, h->stream));
     U.params = as_global(params);
     double vol = 1.0;
@@ -1133,7 +1124,6 @@ int agimp_learned_advance(agimp_handle* h, float dt, int32_t n) {
     }
     h->fixed_chain_len = h->window;
   }
-  const unsigned long long win_init[2] = {~0ull, 0ull};  // CNT_FIRSTVIOL, CNT_STEP
   int t = 0;
   while (t < n) {
     rc = learn_rebuild(h, dt);
@@ -1141,9 +1131,9 @@ int agimp_learned_advance(agimp_handle* h, float dt, int32_t n) {
     h->lstats[AGIMP_LSTAT_REBUILDS] += 1;
     const int w = std::min(h->window, n - t);
     const int par0 = h->host_parity;
-    rc = snapshot(h, par0, true);
-    if (rc) return rc;
-    HIPCHK(h, hipMemcpyAsync(h->hd.cnt + CNT_FIRSTVIOL, win_init, sizeof(win_init), hipMemcpyHostToDevice, h->stream));
+    launch_window_start(h->stream, h->dd, h->snap_cnt);
+    // the window start's particles stay in buffer par0 through the first substep, then in the
+    // spare buffer, which takes par0's place (the window graph swaps them itself)
     if (w > 1 && w == h->fixed_chain_len) {
       HIPCHK(h, hipGraphLaunch(h->fixed_chain[h->host_parity].exec, h->stream));
       h->host_parity ^= w & 1;
@@ -1151,19 +1141,26 @@ int agimp_learned_advance(agimp_handle* h, float dt, int32_t n) {
       for (int i = 0; i < w; ++i) {
         HIPCHK(h, hipGraphLaunch(h->fixed[h->host_parity].exec, h->stream));
         h->host_parity ^= 1;
+        if (i == 0 && w > 1) launch_swap_spare(h->stream, h->dd, par0);
       }
     }
-    unsigned long long st[3], v0 = 0;  // CNT_VIOL, CNT_FIRSTVIOL, CNT_STEP; CNT_VIOL at the window start
-    HIPCHK(h, hipMemcpyAsync(st, h->hd.cnt + CNT_VIOL, sizeof(st), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(h, hipMemcpyAsync(&v0, h->snap_cnt + CNT_VIOL, sizeof(v0), hipMemcpyDeviceToHost, h->stream));
+    if (w > 1) std::swap(h->hd.P[par0], h->hd.spare);  // the host copy follows the device's
+    launch_window_end(h->stream, h->dd, h->snap_cnt, h->win_out_dev);
+    HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
+    // failed checks, first failed substep, substeps; failed checks at the window start
+    const unsigned long long st[3] = {h->win_out[0], h->win_out[1], h->win_out[2]}, v0 = h->win_out[3];
     if (st[1] <= (unsigned long long)w) {
       // the check failed before substep st[1] of the window: keep the st[1]-1 substeps that
-      // ran on a valid octree, then rebuild from there
+      // ran on a valid octree, then rebuild from there.  Back to the window start: its buffer
+      // returns to par0, and the counters are restored.
       h->lstats[AGIMP_LSTAT_VIOLATIONS] += (int64_t)(st[0] - v0);
       const int ok = (int)st[1] - 1;
-      rc = snapshot(h, par0, false);
-      if (rc) return rc;
+      if (w > 1) {
+        launch_swap_spare(h->stream, h->dd, par0);
+        std::swap(h->hd.P[par0], h->hd.spare);
+      }
+      HIPCHK(h, hipMemcpyAsync(h->hd.cnt, h->snap_cnt, NCOUNTERS * 8, hipMemcpyDeviceToDevice, h->stream));
       h->host_parity = par0;
       for (int i = 0; i < ok; ++i) {
         HIPCHK(h, hipGraphLaunch(h->fixed[h->host_parity].exec, h->stream));

@@ -153,6 +153,10 @@ struct SortBufs {
 struct SceneDesc {
   Topo T;
   PBuf P[2];
+  // a third particle buffer (learned prediction only): a window of substeps on one predicted
+  // octree swaps the window start's buffer with it after the first substep, so that state is
+  // kept for a rollback without a copy
+  PBuf spare;
   SortBufs S;
   Physics ph;
   int n;

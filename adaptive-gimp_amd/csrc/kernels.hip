@@ -2333,6 +2333,47 @@ __global__ void k_stress(const SceneDesc* __restrict__ Dsc, int par) {
     for (int j = i; j < D; ++j) P.f[(size_t)(PL::TAU + tau_idx<D>(i, j)) * P.stride + k] = (float)tau[i][j];
 }
 
+// ------------------------------------------------------------------------------ learned windows
+// (single scene) swap particle buffer p with the spare one in the descriptor; later launches
+// read the descriptor anew
+__global__ void k_swap_spare(SceneDesc* Dsc, int p) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  SceneDesc& sd = Dsc[0];
+  const PBuf t = sd.P[p];
+  sd.P[p] = sd.spare;
+  sd.spare = t;
+}
+
+// window start: the counters saved to snap, then the refinement check's first failed substep
+// (none) and the substep count reset
+__global__ void k_window_start(const SceneDesc* Dsc, unsigned long long* snap) {
+  unsigned long long* cnt = Dsc[0].cnt;
+  const int i = threadIdx.x;
+  if (blockIdx.x != 0 || i >= NCOUNTERS) return;
+  const unsigned long long v = cnt[i];
+  snap[i] = v;
+  cnt[i] = (i == CNT_FIRSTVIOL) ? ~0ull : (i == CNT_STEP ? 0ull : v);
+}
+
+// window end: failed checks, first failed substep, substeps, and the failed checks at the
+// window start, written to `out` (host memory mapped for the device)
+__global__ void k_window_end(const SceneDesc* Dsc, const unsigned long long* snap, unsigned long long* out) {
+  const unsigned long long* cnt = Dsc[0].cnt;
+  const int i = threadIdx.x;
+  if (blockIdx.x != 0 || i >= 4) return;
+  out[i] = (i < 3) ? cnt[CNT_VIOL + i] : snap[CNT_VIOL];
+}
+
+void launch_swap_spare(hipStream_t s, SceneDesc* d, int p) {
+  AGIMP_LAUNCH("k_swap_spare", k_swap_spare, dim3(1), dim3(64), 0, s, d, p);
+}
+void launch_window_start(hipStream_t s, const SceneDesc* d, unsigned long long* snap) {
+  AGIMP_LAUNCH("k_window_start", k_window_start, dim3(1), dim3(64), 0, s, d, snap);
+}
+void launch_window_end(hipStream_t s, const SceneDesc* d, const unsigned long long* snap, unsigned long long* out) {
+  AGIMP_LAUNCH("k_window_end", k_window_end, dim3(1), dim3(64), 0, s, d, snap, out);
+}
+
 // ------------------------------------------------------------------------------ host I/O
 // caller order (AoS, like the reference's to_numpy layout) -> slot k = original index
 template <int D>

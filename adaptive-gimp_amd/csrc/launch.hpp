@@ -72,6 +72,12 @@ void launch_download(hipStream_t s, const SceneDesc* d, int par, int dim, int n,
 void launch_dense_level(hipStream_t s, const SceneDesc* d, const Topo& T, int l, int* cell, float* m, float* v);
 void launch_dense_node(hipStream_t s, const SceneDesc* d, const Topo& T, int* out);
 
+// learned windows (kernels.hip), single scene: swap particle buffer p with the spare one; save
+// the counters and reset the check's; report the check's counters to host-mapped `out` [4]
+void launch_swap_spare(hipStream_t s, SceneDesc* d, int p);
+void launch_window_start(hipStream_t s, const SceneDesc* d, unsigned long long* snap);
+void launch_window_end(hipStream_t s, const SceneDesc* d, const unsigned long long* snap, unsigned long long* out);
+
 // learned refinement prediction (learned.hip), single scene
 void launch_unit_features(hipStream_t s, const SceneDesc* d, int dim, int n, int par, const LearnDesc& U);
 void launch_unit_predict(hipStream_t s, const LearnDesc& U, int L, int* cls_out);
