@@ -377,23 +377,12 @@ __global__ void k_bucket_prep(const SceneDesc* __restrict__ Dsc) {
 }
 
 // ------------------------------------------------------------------------------ activation
-// adaptive_gimp.py:87-96: OR ACTIVATED into cell_mask[g_p][base + dI], dI in {0,1}^d
+// adaptive_gimp.py:87-96: OR ACTIVATED into cell_mask[g_p][base + dI], dI in {0,1}^d, for
+// the particle at x with request level g (live: g in [0, L)) and level descriptor Lg.  Every
+// lane of the wavefront calls it (the duplicate test reads the left lane).
 template <int D>
-__global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
-  AGIMP_SCENE
-  if (sd.static_mode) return;
-  const PBuf& P = sd.P[par];
-  const int n = sd.n;
-  unsigned long long* cnt = sd.cnt;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  // the position is loaded beside the request level, not after the test on it
-  float x[3] = {0.f, 0.f, 0.f};
-  if (k < n)
-#pragma unroll
-    for (int a = 0; a < D; ++a) x[a] = pload(pfield(P, a), (uint32_t)k * 4u);
-  const int g = k < n ? P.g[k] : -1;
-  const bool live = g >= 0 && g < T.L;
-  const LevelDesc Lg = T.lv[live ? g : 0];
+__device__ __forceinline__ void activate_requests(const Topo& T, unsigned long long* cnt, const LevelDesc& Lg,
+                                                  const float* x, int g, bool live) {
   int b[3] = {0, 0, 0};
   if (live)
 #pragma unroll
@@ -442,6 +431,32 @@ __global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
     if (dup) continue;
     Lg.cell[q] = ACTIVATED;  // cells only ever receive ACTIVATED in this phase
     if ((t & ~cross) == 0) T.cflag[Lg.gbase + (int)(q / Pow4<D>::v)] = 1;
+  }
+}
+
+template <int D>
+__global__ void k_activate_dynamic(const SceneDesc* __restrict__ Dsc, int par) {
+  AGIMP_SCENE
+  if (sd.static_mode) return;
+  const PBuf& P = sd.P[par];
+  const int n = sd.n;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  // the position is loaded beside the request level, not after the test on it
+  float x[3] = {0.f, 0.f, 0.f};
+  if (k < n)
+#pragma unroll
+    for (int a = 0; a < D; ++a) x[a] = pload(pfield(P, a), (uint32_t)k * 4u);
+  const int g = k < n ? P.g[k] : -1;
+  const bool live = g >= 0 && g < T.L;
+  // particles are sorted by level, so a wavefront's live lanes nearly always request one
+  // level: its descriptor then comes from scalar loads, not a per-lane load round
+  const int g0 = __builtin_amdgcn_readfirstlane(live ? g : -1);
+  if (__ballot(live && g != g0) == 0ull) {
+    const LevelDesc Lg = T.lv[g0 >= 0 ? g0 : 0];
+    activate_requests<D>(T, sd.cnt, Lg, x, g, live);
+  } else {
+    const LevelDesc Lg = T.lv[live ? g : 0];
+    activate_requests<D>(T, sd.cnt, Lg, x, g, live);
   }
 }
 
