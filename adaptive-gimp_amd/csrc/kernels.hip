@@ -967,6 +967,31 @@ __device__ __forceinline__ void lm_probe(const Topo& T, const float (*x)[3], uin
   }
 }
 
+// The particle's (level, leaf block) at its level fl (descriptor Lf): its global block id, or
+// -1 with an out-of-range count if its stencil leaves the padded box; stores its cell inside
+// the block (the P2G sort key)
+template <int D>
+__device__ __forceinline__ int lm_block(const SceneDesc& sd, const LevelDesc& Lf, int k, const float* x) {
+  int J[3];
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    J[a] = base_cell(x[a], Lf, sd.T.pow2) + PAD;
+    ok = ok && J[a] >= 0 && J[a] + 2 < Lf.E;
+  }
+  if (!ok) {
+    count(sd.cnt, CNT_OOB);
+    return -1;
+  }
+  const int b = (D == 2) ? ((J[0] >> 2) * Lf.NB + (J[1] >> 2))
+                         : (((J[0] >> 2) * Lf.NB + (J[1] >> 2)) * Lf.NB + (J[2] >> 2));
+  int lc = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) lc = lc * 4 + (J[a] & 3);
+  sd.S.lkey[k] = (uint8_t)lc;
+  return Lf.gbase + b;
+}
+
 // The rest of the level map for the particle in slot k at x: the levels left to a cell test,
 // fl, the refinement criterion check (when on), and the block.  Returns the global id of its
 // (level, leaf block), or -1 (frozen: no such level, or its stencil leaves the padded box);
@@ -1028,27 +1053,13 @@ __device__ __forceinline__ int lm_finish(const SceneDesc& sd, const PBuf& P, int
     count(cnt, CNT_VIOL);
     atomicMin(&cnt[CNT_FIRSTVIOL], cnt[CNT_STEP]);
   }
-  if (fl >= 0) {
-    const LevelDesc Lf = T.lv[fl];
-    int J[3];
-    bool ok = true;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      J[a] = base_cell(x[a], Lf, T.pow2) + PAD;
-      ok = ok && J[a] >= 0 && J[a] + 2 < Lf.E;
-    }
-    if (!ok) {
-      count(cnt, CNT_OOB);
-    } else {
-      int b = (D == 2) ? ((J[0] >> 2) * Lf.NB + (J[1] >> 2))
-                       : (((J[0] >> 2) * Lf.NB + (J[1] >> 2)) * Lf.NB + (J[2] >> 2));
-      gbid = Lf.gbase + b;
-      // the base cell inside the leaf block: the P2G sort key
-      int lc = 0;
-#pragma unroll
-      for (int a = 0; a < D; ++a) lc = lc * 4 + (J[a] & 3);
-      sd.S.lkey[k] = (uint8_t)lc;
-    }
+  // particles are sorted by level, so a wavefront's mapped lanes nearly always share fl: its
+  // descriptor then comes from scalar loads, not a per-lane load round
+  const int fl0 = __builtin_amdgcn_readfirstlane(fl);
+  if (__ballot(fl >= 0 && fl != fl0) == 0ull) {
+    if (fl >= 0) gbid = lm_block<D>(sd, T.lv[fl0], k, x);
+  } else {
+    if (fl >= 0) gbid = lm_block<D>(sd, T.lv[fl], k, x);
   }
   return gbid;
 }
