@@ -297,14 +297,26 @@ def test_autotune_batch_picks_fastest_recorded_chunk(pkg):
         s.close()
 
 
-@pytest.mark.parametrize("n", [8, 11, 19])
-def test_chained_replays_equal_single_replays(pkg, O, n):
-    """advance() replays chains of 8 substeps per graph launch, then single substeps: the result
-    equals n single eager substeps up to float32 summation order, and octrees rebuilt from both
-    snapshots agree bit for bit."""
-    wl = pkg.scenes.small_3d(seed=3, n=6000, L=3, c=16)
-    a = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, wl.n, wl.initializer(), **wl.params)
-    e = pkg.AdaptiveGIMP(3, wl.level, wl.coarsest_size, wl.n, wl.initializer(), **wl.params)
+def _chain_scene(pkg, name):
+    S = pkg.scenes
+    if name == "small3d":
+        return S.small_3d(seed=3, n=6000, L=3, c=16)
+    if name == "c2_static_small":  # static octree, 4 levels, 2D
+        return S.c2_static_band(seed=1, n=20_000, rho=1000.0)
+    return S.c1_two_squares(rho=1000.0)  # dynamic, 2D
+
+
+@pytest.mark.parametrize("name,n", [("small3d", 8), ("small3d", 11), ("small3d", 19), ("c2_static_small", 17),
+                                    ("c1", 16)])
+def test_chained_replays_equal_single_replays(pkg, O, name, n):
+    """advance() replays chains of 8 substeps per graph launch, then single substeps; inside a
+    chain the P2G and G2P launches clear the octree for the next rebuild and the T and ghost
+    marks share the level map's launch.  The result equals n single eager substeps up to
+    float32 summation order, and octrees rebuilt from both snapshots agree bit for bit."""
+    wl = _chain_scene(pkg, name)
+    d = wl.dim
+    a = pkg.AdaptiveGIMP(d, wl.level, wl.coarsest_size, wl.n, wl.initializer(), wl.grid_initializer, **wl.params)
+    e = pkg.AdaptiveGIMP(d, wl.level, wl.coarsest_size, wl.n, wl.initializer(), wl.grid_initializer, **wl.params)
     a.advance(wl.dt, n)
     for _ in range(n):
         e.substep(wl.dt)
