@@ -38,6 +38,10 @@ constexpr uint8_t ACTIVATED = 1, GHOST = 2, T_JUNCTION = 4, CULLING = 8;
 // not in the reference: a cell without ACTIVATED whose whole volume is covered by finer
 // activated cells (set only by the rebuild of a predicted octree, read by the criterion check)
 constexpr uint8_t COVERED = 16;
+// corner map entries (a level's corner N is shared by its cells N - d, d in {0,1}^d):
+// CORNER_HIT, one of those cells is exactly ACTIVATED; CORNER_FULL (predicted octrees only),
+// every one of them that lies in the padded box holds ACTIVATED or COVERED
+constexpr uint8_t CORNER_HIT = 1, CORNER_FULL = 2;
 
 struct LevelDesc {
   int n;        // cells per axis in the domain, c * 2^l

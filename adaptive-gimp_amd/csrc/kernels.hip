@@ -619,7 +619,7 @@ __device__ __forceinline__ void mark_level_corners(const Topo& T, int l, const i
       qq += t[a][k];
     }
     if (!in) continue;
-    Ll.corner[qq] = 1;
+    Ll.corner[qq] = CORNER_HIT;
     if (other) T.cflag[Ll.gbase + (int)(qq / Pow4<D>::v)] = 1;
   }
 }
@@ -911,6 +911,50 @@ __global__ void k_cover(const SceneDesc* __restrict__ Dsc, int l) {
   }
 }
 
+// After the coverage marks of a predicted octree: CORNER_FULL into the corner entries of every
+// level whose in-box cells around the corner (cells N - d, d in {0,1}^d) all hold ACTIVATED or
+// COVERED, the level map's fast test of the refinement criterion.  Cell N is one of them, so
+// such corners lie in blocks with written cells (cflag set), and are cleared with those blocks.
+// One thread per cell index of a flagged block; each corner entry has one writer.
+template <int D>
+__global__ void k_full_corners(const SceneDesc* __restrict__ Dsc) {
+  AGIMP_SCENE
+  constexpr int CB = Pow4<D>::v;  // cells per block
+  const int64_t tot = (int64_t)T.total_blocks * CB;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    const int g = (int)(t / CB);
+    if (!T.cflag[g]) continue;
+    const int l = level_of_gbid(T, g);
+    const LevelDesc Ll = T.lv[l];
+    const int c = (int)(t % CB);
+    int B[3], N[3];
+    block_coords<D>(Ll.NB, g - Ll.gbase, B);
+    bool in = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      N[a] = 4 * B[a] + ((c >> (2 * (D - 1 - a))) & 3);
+      in = in && N[a] < Ll.E;
+    }
+    if (!in) continue;
+    bool full = true;
+#pragma unroll
+    for (int u = 0; u < (1 << D); ++u) {
+      int J[3];
+      bool inb = true;
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        J[a] = N[a] - ((u >> (D - 1 - a)) & 1);
+        inb = inb && J[a] >= 0;
+      }
+      if (inb) full = full && (Ll.cell[bidx<D>(Ll.NB, J)] & (ACTIVATED | COVERED));
+    }
+    if (full) {
+      const uint32_t q = bidx<D>(Ll.NB, N);
+      Ll.corner[q] = Ll.corner[q] | CORNER_FULL;
+    }
+  }
+}
+
 // level-l cell I lies in a unit (cell of level ulev <= l) whose predicted level is >= l
 template <int D>
 __device__ __forceinline__ bool unit_covers(const Topo& T, int ulev, const int8_t* upred, int l, const int* I) {
@@ -1004,7 +1048,7 @@ __device__ __forceinline__ int lm_finish(const SceneDesc& sd, const PBuf& P, int
   int fl = -1, gbid = -1;
   bool hit[NL];
 #pragma unroll
-  for (int i = 0; i < NL; ++i) hit[i] = ((inb >> i) & 1u) && hv[i] != 0u;
+  for (int i = 0; i < NL; ++i) hit[i] = ((inb >> i) & 1u) && (hv[i] & CORNER_HIT);
   if (slow) {
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
@@ -1029,9 +1073,10 @@ __device__ __forceinline__ int lm_finish(const SceneDesc& sd, const PBuf& P, int
     const int l = T.L - 1 - i;
     if (l < 0) continue;
     if (hit[i] && fl < 0) fl = l;
-    if (l == gp) {
-      // the criterion check: the in-box request cells of level gp must hold ACTIVATED or
-      // COVERED, or lie in a unit predicted at gp or finer
+    // the criterion check: the in-box request cells of level gp must hold ACTIVATED or
+    // COVERED, or lie in a unit predicted at gp or finer; a corner entry with CORNER_FULL
+    // answers the first test for all of them
+    if (l == gp && !(((inb >> i) & 1u) && (hv[i] & CORNER_FULL))) {
       const LevelDesc Ll = T.lv[l];
       int b[3];
 #pragma unroll
@@ -2544,8 +2589,10 @@ void launch_refine(hipStream_t s, const SceneDesc* d, const Dims& m, int par, co
     // chained: the T and ghost marks run in the bucketing pass's first launch, beside the level map
     if (!chained) AGIMP_LAUNCH("k_mark_TG", k_mark_TG<DD>, lg, dim3(256), 0, s, d);
     if (!chained) AGIMP_LAUNCH("k_copy_flags", k_copy_flags, fg, dim3(256), 0, s, d, 1);
-    if (learn)
+    if (learn) {
       for (int l = m.maxL - 2; l >= 0; --l) AGIMP_LAUNCH("k_cover", k_cover<DD>, dim3(2048), dim3(256), 0, s, d, l);
+      AGIMP_LAUNCH("k_full_corners", k_full_corners<DD>, dim3(2048), dim3(256), 0, s, d);
+    }
   });
 }
 
